@@ -1,0 +1,124 @@
+"""Device side of the structured layout (``layout.StructuredLayout``).
+
+``HipGrid`` uploads the renumbering, the regular-row flags and the composite
+rows of one mesh, and exposes the elliptic operator and the PCG pieces of
+``Driver.helm_solve`` (``ib_driver_np.py:311-410``) on the internal numbering.
+The driver moves the right-hand side and the warm start into this numbering
+once per solve and the solution back, so every PCG iteration runs on dense,
+coalesced edge and cell blocks.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .core import dev_stream, ptr
+from .layout import MAX_LEVELS, build_layout, dir_mask_internal
+
+
+class HipGrid:
+    def __init__(self, core, precisions=("float64",), margin=2, layout=None):
+        self.core = core
+        dev = self.device = core.device
+        self.lay = lay = layout if layout is not None else build_layout(core.mesh, core.geo, core.tables,
+                                                                        margin=margin)
+        self.ncell, self.nedge = lay.ncell, lay.nedge
+        t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)
+        self._cell_reg = t(lay.reg_cell, torch.uint8)
+        self._edge_reg = t(lay.reg_edge, torch.uint8)
+        self._dir = t(np.ones(lay.nedge, np.uint8), torch.uint8)
+        # reference -> internal gather: idx over internal slots; internal -> reference: idx over ref edges
+        self._to_int_idx = t(lay.edge_inv, torch.int64)
+        self._to_int_sig = t(lay.sigma_int, torch.int8)
+        self._to_ref_idx = t(lay.edge_map, torch.int64)
+        self._to_ref_sig = t(lay.edge_sigma, torch.int8)
+        self._comp = {}
+        for name in ("g1", "vis", "g3"):
+            c = getattr(lay, name)
+            self._comp[name] = {"rows": t(c.rows, torch.int32), "ptr": t(c.ptr, torch.int32),
+                                "col": t(c.col, torch.int32), "n": int(c.rows.shape[0]), "coef64": c.coef}
+        self._coef = {}
+        self._struct = {}
+        self.work = {}
+        for prec in precisions:
+            self.add_precision(prec)
+
+    def add_precision(self, prec):
+        if prec in self._struct:
+            return
+        dt = torch.float64 if prec == "float64" else torch.float32
+        self._coef[prec] = {n: torch.from_numpy(np.ascontiguousarray(c["coef64"])).to(self.device, dt)
+                            for n, c in self._comp.items()}
+        w = {"face": torch.empty(1, dtype=dt, device=self.device),
+             "cell_a": torch.zeros(3 * self.ncell, dtype=dt, device=self.device),
+             "cell_b": torch.zeros(3 * self.ncell, dtype=dt, device=self.device),
+             "partials": torch.zeros(8 * N.MAX_PARTIALS, dtype=torch.float64, device=self.device)}
+        w["struct"] = N.Work(ptr(w["face"]), ptr(w["cell_a"]), ptr(w["cell_b"]), ptr(w["partials"]))
+        self.work[prec] = w
+        self._struct[prec] = self._make(prec)
+
+    def _make(self, prec):
+        g = N.Grid()
+        g.dtype = N.F64 if prec == "float64" else N.F32
+        lay = self.lay
+        g.nlevels = len(lay.levels)
+        if g.nlevels > MAX_LEVELS:
+            raise ValueError("too many levels")
+        g.ncell, g.nedge = lay.ncell, lay.nedge
+        for i, lv in enumerate(lay.levels):
+            L = g.lev[i]
+            L.nx, L.ny, L.nz = lv.nx, lv.ny, lv.nz
+            L.cell_off = lv.cell_off
+            L.e_off[0], L.e_off[1], L.e_off[2] = lv.e_off
+            L.h = lv.h
+        g.cell_reg, g.edge_reg, g.dir = ptr(self._cell_reg), ptr(self._edge_reg), ptr(self._dir)
+        c, k = self._comp, self._coef[prec]
+        g.g1_nrows, g.g1_row, g.g1_ptr, g.g1_col, g.g1_coef = (c["g1"]["n"], ptr(c["g1"]["rows"]), ptr(c["g1"]["ptr"]),
+                                                                ptr(c["g1"]["col"]), ptr(k["g1"]))
+        g.v_nrows, g.v_row, g.v_ptr, g.v_col, g.v_coef = (c["vis"]["n"], ptr(c["vis"]["rows"]), ptr(c["vis"]["ptr"]),
+                                                           ptr(c["vis"]["col"]), ptr(k["vis"]))
+        g.g3_nrows, g.g3_row, g.g3_ptr, g.g3_col, g.g3_coef = (c["g3"]["n"], ptr(c["g3"]["rows"]), ptr(c["g3"]["ptr"]),
+                                                                ptr(c["g3"]["col"]), ptr(k["g3"]))
+        return g
+
+    def set_dir_mask(self, dir_mask_ref):
+        self._dir.copy_(torch.from_numpy(dir_mask_internal(self.lay, dir_mask_ref)))
+
+    def struct(self, prec="float64"):
+        return C.byref(self._struct[prec])
+
+    def work_struct(self, prec="float64"):
+        return C.byref(self.work[prec]["struct"])
+
+    # ------------------------------------------------------- numbering
+    def to_internal(self, x_ref, out=None, dtype=torch.float64):
+        out = torch.empty(self.nedge, dtype=dtype, device=self.device) if out is None else out
+        N.call("pyib_permute", _code(x_ref.dtype), ptr(x_ref), _code(out.dtype), ptr(out), C.c_int64(self.nedge),
+               ptr(self._to_int_idx), ptr(self._to_int_sig), dev_stream())
+        return out
+
+    def to_reference(self, x_int, out=None, dtype=torch.float64):
+        n = self.core.nedge
+        out = torch.empty(n, dtype=dtype, device=self.device) if out is None else out
+        N.call("pyib_permute", _code(x_int.dtype), ptr(x_int), _code(out.dtype), ptr(out), C.c_int64(n),
+               ptr(self._to_ref_idx), ptr(self._to_ref_sig), dev_stream())
+        return out
+
+    def internal_from_host(self, x_ref):
+        v = np.zeros(self.nedge)
+        v[self.lay.edge_map] = self.lay.edge_sigma * np.asarray(x_ref)
+        return torch.from_numpy(v).to(self.device)
+
+    def flow_mat(self, p_int, rou, visc, dt, out=None):
+        prec = "float64" if p_int.dtype == torch.float64 else "float32"
+        out = torch.zeros(self.nedge, dtype=p_int.dtype, device=self.device) if out is None else out
+        N.call("pyib_grid_flow_mat", self.struct(prec), self.work_struct(prec), float(rou), float(visc),
+               0.5 * float(dt), ptr(p_int), ptr(out), C.c_void_p(0), dev_stream())
+        return out
+
+
+def _code(dtype):
+    return N.F64 if dtype == torch.float64 else N.F32

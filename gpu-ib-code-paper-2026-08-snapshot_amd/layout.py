@@ -1,0 +1,366 @@
+"""Structured per-level layout of the elliptic operator (host setup, NumPy).
+
+Each 2:1 level L is a box of (nx, ny, nz) cells of spacing h_L.  The CG
+unknowns are renumbered level by level into three dense edge blocks
+(x-edges (nx, ny+1, nz+1), y-edges (nx+1, ny, nz+1), z-edges (nx+1, ny+1, nz))
+and cell vectors into (nx, ny, nz) blocks, all with the +axis orientation
+(``sigma`` = +-1 maps PyIB's edge orientation, node order of E2N, onto it).
+Slots of the box that are not mesh elements (the hole under a finer level,
+coarse edges collapsed onto fine halves) stay zero.
+
+Away from level boundaries the operator A = rot.integr.V.interp.curl
+(ib_core_np.py:213-220) is a constant-coefficient stencil: the "regular"
+rows.  They are cells at least ``margin`` cells from the box boundary and
+from the hole, and edges whose four adjacent cells are regular.  A regular
+row is kept only if the mesh geometry around it equals the closed form
+(rf1 = +-h/2 e_d, AoL = h, vol_ci = 1/h^3, Af = h^2, 4 unit-sign links) to
+the tolerance of the arithmetic precision; otherwise it is demoted.  Every
+other row ("irregular": level interfaces, the outer boundary) is an explicit
+composite row built here from the reference geometry:
+
+* G1 (cell <- edges): vol_ci * (sum_c1 rf1 U - sum_c2 rf2 U), U = curl p
+* V  (cell <- cells): x - ...: coefficients vol_ci*AoL per face, diagonal
+  -vol_ci*sum AoL (zero ghost), applied as y = rou x - (hdt*visc) V x
+* G3 (edge <- cells): rot(integr(.)) with the two owners' rf1/-rf2
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+MAX_LEVELS = 8
+
+
+@dataclass
+class Level:
+    nx: int
+    ny: int
+    nz: int
+    h: float
+    lo: tuple
+    hole_lo: tuple
+    hole_hi: tuple
+    cell_off: int
+    e_off: tuple
+    ncell: int = 0
+    nedge: int = 0
+
+
+@dataclass
+class Composite:
+    rows: np.ndarray            # internal row ids
+    ptr: np.ndarray
+    col: np.ndarray
+    coef: np.ndarray            # (nnz,) or (nnz,3)
+
+
+@dataclass
+class StructuredLayout:
+    levels: list
+    ncell: int                  # internal (padded) cells
+    nedge: int                  # internal (padded) edges
+    cell_map: np.ndarray        # reference cell -> internal
+    edge_map: np.ndarray        # reference edge -> internal
+    edge_sigma: np.ndarray      # +-1
+    edge_inv: np.ndarray        # internal -> reference edge (-1: empty slot)
+    sigma_int: np.ndarray       # +-1 on internal slots (1 on empty)
+    reg_cell: np.ndarray        # uint8 over internal cells
+    reg_edge: np.ndarray        # uint8 over internal edges
+    g1: Composite = None
+    vis: Composite = None
+    g3: Composite = None
+    stats: dict = field(default_factory=dict)
+
+
+def _edge_shape(lv, d):
+    n = [lv.nx, lv.ny, lv.nz]
+    return tuple(n[a] + (0 if a == d else 1) for a in range(3))
+
+
+def _cells_of_level(lv):
+    m = np.ones((lv.nx, lv.ny, lv.nz), bool)
+    a, b = lv.hole_lo, lv.hole_hi
+    m[a[0]:b[0], a[1]:b[1], a[2]:b[2]] = False
+    return m
+
+
+def build_layout(mesh, geo, tables, margin=2, tol=1e-13):
+    """Internal layout + classification.  ``tol`` is the relative tolerance of the
+    closed-form coefficient check (0: bitwise).  The default keeps rows whose
+    geometry differs from the closed form only by the rounding of non-dyadic
+    node coordinates (~1e-16 relative)."""
+    N = mesh.n_layers
+    if N > MAX_LEVELS:
+        raise ValueError(f"at most {MAX_LEVELS} levels")
+    levels = []
+    coff, eoff = 0, 0
+    for L in range(N):
+        nx, ny, nz = mesh.layer_shape[L]
+        if L > 0:
+            hl = tuple(int(v) for v in mesh.layer_hole_lo[L])
+            hs = tuple(v // 2 for v in mesh.layer_shape[L - 1])
+            hh = tuple(hl[a] + hs[a] for a in range(3))
+        else:
+            hl, hh = (0, 0, 0), (0, 0, 0)
+        lv = Level(nx, ny, nz, float(mesh.layer_spacing[L]), tuple(mesh.layer_min[L]), hl, hh, coff, (0, 0, 0))
+        lv.ncell = nx * ny * nz
+        offs = []
+        for d in range(3):
+            offs.append(eoff)
+            eoff += int(np.prod(_edge_shape(lv, d)))
+        lv.e_off = tuple(offs)
+        coff += lv.ncell
+        levels.append(lv)
+    ncell_i, nedge_i = coff, eoff
+    if max(ncell_i, nedge_i) >= 2 ** 31 - 1:
+        raise ValueError("mesh too large for int32 internal indices")
+    # ---- cells: reference layer order = C order of the live cells of each level
+    cell_map = np.empty(geo.ncell, np.int64)
+    start = 0
+    for L, lv in enumerate(levels):
+        live = np.flatnonzero(_cells_of_level(lv).ravel())
+        n = int(mesh.layer_cell_counts[L])
+        if live.shape[0] != n:
+            raise ValueError("layer cell count does not match its box")
+        cell_map[start:start + n] = lv.cell_off + live
+        start += n
+    # ---- edges: level from length, direction from the tangent, slot from the lower node
+    nodes, E2N = mesh.nodes, mesh.edges_to_nodes
+    te = geo.te
+    d = np.argmax(np.abs(te), axis=1)
+    length = np.abs(te[np.arange(te.shape[0]), d])
+    h0 = levels[0].h
+    lev = np.rint(np.log2(length / h0)).astype(np.int64)
+    if lev.min() < 0 or lev.max() >= N or not np.allclose(length, h0 * 2.0 ** lev, rtol=1e-9):
+        raise ValueError("edge length is not a level spacing")
+    sigma = np.where(te[np.arange(te.shape[0]), d] > 0, 1, -1).astype(np.int8)
+    lower = np.where((sigma > 0)[:, None], nodes[E2N[:, 0]], nodes[E2N[:, 1]])
+    edge_map = np.empty(te.shape[0], np.int64)
+    for L, lv in enumerate(levels):
+        sel = lev == L
+        ijk = np.rint((lower[sel] - np.asarray(lv.lo)) / lv.h).astype(np.int64)
+        dd = d[sel]
+        out = np.empty(int(sel.sum()), np.int64)
+        for a in range(3):
+            s = dd == a
+            shp = _edge_shape(lv, a)
+            q = ijk[s]
+            if (q < 0).any() or (q >= np.asarray(shp)).any():
+                raise ValueError("edge outside its level box")
+            out[s] = lv.e_off[a] + (q[:, 0] * shp[1] + q[:, 1]) * shp[2] + q[:, 2]
+        edge_map[sel] = out
+    if np.unique(edge_map).shape[0] != edge_map.shape[0]:
+        raise ValueError("two edges map to one slot")
+    edge_inv = np.full(nedge_i, -1, np.int64)
+    edge_inv[edge_map] = np.arange(edge_map.shape[0])
+    sigma_int = np.ones(nedge_i, np.int8)
+    sigma_int[edge_map] = sigma
+    lay = StructuredLayout(levels, ncell_i, nedge_i, cell_map, edge_map, sigma, edge_inv, sigma_int,
+                           np.zeros(ncell_i, np.uint8), np.zeros(nedge_i, np.uint8))
+    _classify(lay, margin)
+    _verify_regular(lay, mesh, geo, tables, tol)
+    _composites(lay, geo, tables)
+    return lay
+
+
+def _reg_cells_level(lv, m):
+    r = np.zeros((lv.nx, lv.ny, lv.nz), bool)
+    r[m:lv.nx - m, m:lv.ny - m, m:lv.nz - m] = True
+    a, b = lv.hole_lo, lv.hole_hi
+    if b[0] > a[0]:
+        r[max(a[0] - m, 0):b[0] + m, max(a[1] - m, 0):b[1] + m, max(a[2] - m, 0):b[2] + m] = False
+    return r
+
+
+def _reg_edges_level(lv, rc, d):
+    """An edge is regular when its four adjacent cells are regular."""
+    shp = _edge_shape(lv, d)
+    out = np.zeros(shp, bool)
+    o = [a for a in range(3) if a != d]
+    # cells adjacent to edge (i,j,k) along d: offsets -1/0 on the two other axes
+    pad = np.zeros(tuple(s + 2 for s in rc.shape), bool)
+    pad[1:-1, 1:-1, 1:-1] = rc
+    acc = np.ones(shp, bool)
+    for s0 in (-1, 0):
+        for s1 in (-1, 0):
+            sl = [slice(1, 1 + shp[0]), slice(1, 1 + shp[1]), slice(1, 1 + shp[2])]
+            for ax, sft in ((o[0], s0), (o[1], s1)):
+                sl[ax] = slice(1 + sft, 1 + sft + shp[ax])
+            acc &= pad[tuple(sl)]
+    out[:] = acc
+    return out
+
+
+def _classify(lay, m):
+    for lv in lay.levels:
+        rc = _reg_cells_level(lv, m)
+        lay.reg_cell[lv.cell_off:lv.cell_off + lv.ncell] = rc.ravel()
+        for d in range(3):
+            re = _reg_edges_level(lv, rc, d)
+            n = re.size
+            lay.reg_edge[lv.e_off[d]:lv.e_off[d] + n] = re.ravel()
+
+
+def _close(a, b, tol, scale):
+    """|a - b| <= tol*scale (bitwise equality when tol == 0)."""
+    return np.abs(a - b) <= tol * scale if tol > 0 else a == b
+
+
+def _verify_regular(lay, mesh, geo, tables, tol):
+    """Demote regular rows whose surrounding geometry differs from the closed form."""
+    ncell = geo.ncell
+    cinv = np.full(lay.ncell, -1, np.int64)
+    cinv[lay.cell_map] = np.arange(ncell)
+    lev_of_cell = np.empty(ncell, np.int64)
+    for L, lv in enumerate(lay.levels):
+        sel = (lay.cell_map >= lv.cell_off) & (lay.cell_map < lv.cell_off + lv.ncell)
+        lev_of_cell[sel] = L
+    h = np.array([lv.h for lv in lay.levels])[lev_of_cell]
+    ok_cell = _close(geo.vol_ci, 1.0 / h ** 3, tol, 1.0 / h ** 3)
+    # faces: both-owner cells (or the one owner) must see +-h/2 e_d, AoL = h, Af = h^2, 4 +-1 links
+    F2C = geo.F2C
+    hf = h[F2C[:, 0]]
+    dn = np.argmax(np.abs(geo.nf), axis=1)
+    e_d = np.zeros((geo.nface, 3))
+    e_d[np.arange(geo.nface), dn] = 1.0
+    half = 0.5 * hf[:, None] * e_d
+    sgn = np.sign(geo.rf1[np.arange(geo.nface), dn])
+    ok_face = np.all(_close(np.abs(geo.rf1), half, tol, hf[:, None]), axis=1)
+    ok_face &= np.all(_close(np.abs(geo.rf2), np.where(geo.m1[:, None], half, 0.0), tol, hf[:, None]), axis=1)
+    ok_face &= _close(np.abs(geo.AoL), hf, tol, hf) & _close(geo.Af, hf * hf, tol, hf * hf)
+    ok_face &= geo.m1 & (h[np.maximum(F2C[:, 1], 0)] == hf) & (sgn != 0)
+    ok_face &= np.diff(tables.curl_ptr) == 4
+    bad_face = ~ok_face
+    bad_cell = ~ok_cell.copy()
+    bad_cell[F2C[bad_face, 0]] = True
+    m1b = bad_face & geo.m1
+    bad_cell[F2C[m1b, 1]] = True
+    # a regular cell needs good own geometry and good neighbours (V reads them)
+    demote = np.zeros(lay.ncell, bool)
+    demote[lay.cell_map[bad_cell]] = True
+    nb_bad = np.zeros(ncell, bool)
+    own = np.repeat(np.arange(ncell), np.diff(tables.cf_ptr))
+    oth = tables.cf_other
+    inner = oth < ncell
+    nb_bad_idx = own[inner][bad_cell[oth[inner]]]
+    nb_bad[nb_bad_idx] = True
+    demote[lay.cell_map[nb_bad]] = True
+    before = int(lay.reg_cell.sum())
+    lay.reg_cell[demote] = 0
+    # edges: re-derive from the (possibly demoted) cells
+    for lv in lay.levels:
+        rc = lay.reg_cell[lv.cell_off:lv.cell_off + lv.ncell].reshape(lv.nx, lv.ny, lv.nz).astype(bool)
+        for d in range(3):
+            re = _reg_edges_level(lv, rc, d)
+            seg = slice(lv.e_off[d], lv.e_off[d] + re.size)
+            lay.reg_edge[seg] &= re.ravel().astype(np.uint8)
+    # an edge slot with no mesh edge can never be regular
+    lay.reg_edge[lay.edge_inv < 0] = 0
+    lay.stats.update(regular_cells=int(lay.reg_cell.sum()), demoted_cells=before - int(lay.reg_cell.sum()),
+                     regular_edges=int(lay.reg_edge.sum()))
+
+
+def expand_ranges(starts, counts):
+    """Concatenation of arange(starts[i], starts[i]+counts[i]) for all i (vectorised)."""
+    starts = np.asarray(starts, np.int64)
+    counts = np.asarray(counts, np.int64)
+    total = int(counts.sum())
+    if total == 0:
+        return np.zeros(0, np.int64)
+    first = np.cumsum(counts) - counts
+    return np.arange(total, dtype=np.int64) - np.repeat(first, counts) + np.repeat(starts, counts)
+
+
+def _csr_rows(rows_of_entry, n_rows_total, cols, coefs, row_ids):
+    """Sum duplicate (row, col) entries and pack the rows listed in row_ids (internal ids)."""
+    key = rows_of_entry.astype(np.int64) * (np.int64(1) << 32) + cols.astype(np.int64)
+    order = np.argsort(key, kind="stable")
+    key, coefs = key[order], coefs[order]
+    uniq, start = np.unique(key, return_index=True)
+    summed = np.add.reduceat(coefs, start, axis=0) if coefs.shape[0] else coefs
+    r = (uniq >> 32).astype(np.int64)
+    c = (uniq & 0xFFFFFFFF).astype(np.int64)
+    counts = np.bincount(r, minlength=n_rows_total)
+    full_ptr = np.zeros(n_rows_total + 1, np.int64)
+    np.cumsum(counts, out=full_ptr[1:])
+    sel_counts = counts[row_ids]
+    ptr = np.zeros(row_ids.shape[0] + 1, np.int64)
+    np.cumsum(sel_counts, out=ptr[1:])
+    take = expand_ranges(full_ptr[row_ids], sel_counts)
+    return ptr.astype(np.int32), c[take].astype(np.int32), summed[take]
+
+
+def _composites(lay, geo, t):
+    ncell = geo.ncell
+    cmap, emap, sig = lay.cell_map, lay.edge_map, lay.edge_sigma.astype(np.float64)
+    real_cell = np.zeros(lay.ncell, bool)
+    real_cell[cmap] = True
+    irr_cells_int = np.flatnonzero(real_cell & (lay.reg_cell == 0))
+    cinv = np.full(lay.ncell, -1, np.int64)
+    cinv[cmap] = np.arange(ncell)
+    irr_ref = cinv[irr_cells_int]
+    # cell-face entries of the irregular cells
+    cnt = np.diff(t.cf_ptr)
+    sel = np.zeros(ncell, bool)
+    sel[irr_ref] = True
+    own = np.repeat(np.arange(ncell), cnt)
+    first = np.zeros(t.cf_face.shape[0], bool)
+    pos = np.arange(t.cf_face.shape[0]) - np.repeat(t.cf_ptr[:-1], cnt)
+    first[pos < np.repeat(t.cf_mid - t.cf_ptr[:-1], cnt)] = True
+    ent = np.flatnonzero(sel[own])
+    f_e, own_e, first_e, oth_e = t.cf_face[ent], own[ent], first[ent], t.cf_other[ent]
+    # --- G1: cell <- edge
+    rfc = np.where(first_e[:, None], geo.rf1[f_e], -geo.rf2[f_e]) * geo.vol_ci[own_e][:, None]
+    lcnt = np.diff(t.curl_ptr)[f_e]
+    rep = np.repeat(np.arange(f_e.shape[0]), lcnt)
+    link = expand_ranges(t.curl_ptr[f_e], lcnt)
+    sv = t.curl_idx[link].astype(np.int64)
+    e_ref = np.where(sv >= 0, sv, -sv - 1)
+    s_link = np.where(sv >= 0, 1.0, -1.0)
+    coef = rfc[rep] * (s_link * sig[e_ref])[:, None]
+    ptr, col, cf = _csr_rows(cmap[own_e][rep], lay.ncell, emap[e_ref], coef, irr_cells_int)
+    lay.g1 = Composite(irr_cells_int.astype(np.int32), ptr, col, cf)
+    # --- V (cell <- cell): off-diagonal vol_ci*AoL, diagonal -vol_ci*sum AoL
+    aol = geo.AoL[f_e] * geo.vol_ci[own_e]
+    inner = oth_e < ncell
+    rows = np.concatenate([cmap[own_e][inner], cmap[own_e]])
+    cols = np.concatenate([cmap[oth_e[inner]], cmap[own_e]])
+    vals = np.concatenate([aol[inner], -aol])
+    ptr, col, cf = _csr_rows(rows, lay.ncell, cols, vals, irr_cells_int)
+    lay.vis = Composite(irr_cells_int.astype(np.int32), ptr, col, cf)
+    # --- G3: edge <- cell, for real irregular edges
+    real_edge = lay.edge_inv >= 0
+    irr_edges_int = np.flatnonzero(real_edge & (lay.reg_edge == 0))
+    er = lay.edge_inv[irr_edges_int]
+    rcnt = np.diff(t.rot_ptr)[er]
+    rep = np.repeat(np.arange(er.shape[0]), rcnt)
+    link = expand_ranges(t.rot_ptr[er], rcnt)
+    sv = t.rot_idx[link].astype(np.int64)
+    f_ref = np.where(sv >= 0, sv, -sv - 1)
+    s_link = np.where(sv >= 0, 1.0, -1.0) * sig[er[rep]]
+    c1, c2 = geo.F2C[f_ref, 0], geo.F2C[f_ref, 1]
+    rows = np.concatenate([irr_edges_int[rep], irr_edges_int[rep][c2 >= 0]])
+    cols = np.concatenate([cmap[c1], cmap[c2[c2 >= 0]]])
+    vals = np.concatenate([geo.rf1[f_ref] * s_link[:, None], -geo.rf2[f_ref][c2 >= 0] * s_link[c2 >= 0][:, None]])
+    ptr, col, cf = _csr_rows(rows, lay.nedge, cols, vals, irr_edges_int)
+    lay.g3 = Composite(irr_edges_int.astype(np.int32), ptr, col, cf)
+    lay.stats.update(irregular_cells=int(irr_cells_int.shape[0]), irregular_edges=int(irr_edges_int.shape[0]),
+                     g1_nnz=int(lay.g1.col.shape[0]), v_nnz=int(lay.vis.col.shape[0]), g3_nnz=int(lay.g3.col.shape[0]))
+
+
+def dir_mask_internal(lay, dir_mask_ref):
+    """Dirichlet mask in internal numbering; empty slots count as masked."""
+    m = np.ones(lay.nedge, np.uint8)
+    m[lay.edge_map] = np.asarray(dir_mask_ref, bool).astype(np.uint8)
+    return m
+
+
+def to_internal_edges(lay, x_ref):
+    out = np.zeros(lay.nedge)
+    out[lay.edge_map] = lay.edge_sigma * np.asarray(x_ref)
+    return out
+
+
+def to_reference_edges(lay, x_int):
+    return lay.edge_sigma * np.asarray(x_int)[lay.edge_map]

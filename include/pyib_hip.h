@@ -1,0 +1,239 @@
+/*
+ * pyib_hip.h -- C ABI of the MI355X (gfx950) time-step hot path of PyIB.
+ *
+ * The reference solver has no FFI: its seam is the array backend chosen at
+ * import (src/pyib/solver/backend.py:10-34) plus duck-typed operator objects
+ * (IBCore, Driver, ImmersedBoundary, ClosedSphereLoadCorrection).  Each entry
+ * point below replaces one device-side operation those objects perform through
+ * NumPy/CuPy; the comment on each names the reference call it replaces.
+ *
+ * Conventions
+ *  - All array arguments are device pointers (hipMalloc'd or torch-owned); no
+ *    entry point allocates, frees or synchronises.  Workspaces are passed in.
+ *  - `dtype` selects the floating type of coefficients and vectors:
+ *    PYIB_F64 (double) or PYIB_F32 (float).  Reductions always accumulate and
+ *    return in double.
+ *  - `stream` is a hipStream_t passed as void*; NULL means the default stream.
+ *  - Signed link indices encode the circulation sign: v >= 0 means +x[v],
+ *    v < 0 means -x[-v-1].
+ *  - Every function returns 0 on success and a nonzero code on failure; the
+ *    message is then available from pyib_last_error().
+ */
+#ifndef PYIB_HIP_H
+#define PYIB_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { PYIB_F64 = 0, PYIB_F32 = 1 };
+
+/* Incidence/geometry tables of one mesh, in PyIB's global numbering.
+ * Built once on the host from the reference geometry (ib_core_np.py:68-166). */
+typedef struct pyib_mesh {
+  int32_t dtype;            /* PYIB_F64 or PYIB_F32: type of every float array below */
+  int32_t reserved;
+  int64_t ncell, nface, nedge, nghost;
+  /* curl: face <- edges over the flat face/edge links (ib_core_np.py:178-179) */
+  const int32_t *curl_ptr;  /* nface+1 */
+  const int32_t *curl_idx;  /* signed edge index per link, link order */
+  /* rot: edge <- faces, the transpose of curl (ib_core_np.py:174-176) */
+  const int32_t *rot_ptr;   /* nedge+1 */
+  const int32_t *rot_idx;   /* signed face index */
+  /* cell <- face incidence: for cell c, entries [cf_ptr[c], cf_mid[c]) are the
+   * faces whose first owner is c, [cf_mid[c], cf_ptr[c+1]) those whose second
+   * owner is c, each in face order; cf_other is the opposite owner, or the
+   * ghost index ncell+k for boundary face k (ib_driver_np.py:112-118). */
+  const int32_t *cf_ptr;    /* ncell+1 */
+  const int32_t *cf_mid;    /* ncell */
+  const int32_t *cf_face;
+  const int32_t *cf_other;
+  const int32_t *f2c;       /* nface x 2, second owner -1 on boundary faces */
+  const void *rf1, *rf2;    /* nface x 3 */
+  const void *aol;          /* nface: (rf.nf)/|rf|^2 */
+  const void *vol_ci;       /* ncell: inverse cell volume */
+  /* boundary ghosts, one per boundary face k (ib_driver_np.py:275-291) */
+  const int32_t *gh_face;   /* nghost */
+  const int32_t *gh_cell;   /* first owner */
+  const int32_t *gh_kind;   /* 0 inlet zero-tangential, 1 zero-gradient, 2 wall, 3 lid, 4 spanwise copy */
+  const void *gh_nf;        /* nghost x 3: face area vector */
+  const void *gh_af2;       /* nghost: Af^2 */
+  /* inlet stream-function values (ib_driver_np.py:271) */
+  int64_t ninlet;
+  const int32_t *inlet_edge;
+  const void *inlet_val;
+  const uint8_t *dir_mask;  /* nedge: Dirichlet edges, residual forced to zero */
+} pyib_mesh;
+
+/* Workspaces for one operator application (owned by the caller). */
+typedef struct pyib_work {
+  void *face;               /* nface */
+  void *cell_a;             /* ncell x 3 */
+  void *cell_b;             /* ncell x 3 */
+  double *partials;         /* >= PYIB_MAX_PARTIALS doubles */
+} pyib_work;
+
+#define PYIB_MAX_PARTIALS 2048
+
+/* CG state; vectors have the mesh dtype.  scal is a device array of
+ * PYIB_NSCAL doubles: [eta, eta_new, den, alpha, beta, rr, err_tot, l2_init]. */
+typedef struct pyib_cg {
+  void *x, *r, *p, *w;
+  const void *prec;
+  double *scal;
+} pyib_cg;
+#define PYIB_NSCAL 8
+
+/* Structured per-level layout of the elliptic operator (one box per 2:1 level).
+ * Edge blocks per level: x (nx, ny+1, nz+1), y (nx+1, ny, nz+1), z (nx+1, ny+1, nz),
+ * +axis oriented; cell block (nx, ny, nz); cell vectors component-major. */
+#define PYIB_MAX_LEVELS 8
+typedef struct pyib_level {
+  int32_t nx, ny, nz, reserved;
+  int64_t cell_off;         /* first internal cell of the level */
+  int64_t e_off[3];         /* first internal x/y/z edge of the level */
+  double h;                 /* spacing */
+} pyib_level;
+
+typedef struct pyib_grid {
+  int32_t dtype;            /* PYIB_F64 or PYIB_F32: composite coefficients and vectors */
+  int32_t nlevels;
+  int64_t ncell, nedge;     /* internal (padded) sizes */
+  pyib_level lev[PYIB_MAX_LEVELS];
+  const uint8_t *cell_reg;  /* ncell: row computed by the stencil */
+  const uint8_t *edge_reg;  /* nedge: row computed by the stencil */
+  const uint8_t *dir;       /* nedge: Dirichlet edge or empty slot (residual forced to 0) */
+  /* composite rows of the other (interface/boundary) rows, CSR */
+  int64_t g1_nrows; const int32_t *g1_row, *g1_ptr, *g1_col; const void *g1_coef; /* coef nnz x 3 */
+  int64_t v_nrows;  const int32_t *v_row, *v_ptr, *v_col;  const void *v_coef;    /* coef nnz */
+  int64_t g3_nrows; const int32_t *g3_row, *g3_ptr, *g3_col; const void *g3_coef; /* coef nnz x 3 */
+} pyib_grid;
+
+const char *pyib_last_error(void);
+int pyib_device_count(int *count);
+
+/* ---- incidence operators (ib_core_np.py:174-228) ---- */
+int pyib_curl(const pyib_mesh *m, const void *edge, void *face, void *stream);          /* IBCore.curl */
+int pyib_rot(const pyib_mesh *m, const void *face, void *edge, void *stream);           /* IBCore.rot */
+int pyib_interp_f2c(const pyib_mesh *m, const void *face, void *cellvec, void *stream); /* IBCore.interp_f2c */
+int pyib_integr_c2f(const pyib_mesh *m, const void *cellvec, void *face, void *stream); /* IBCore.integr_c2f */
+int pyib_cdiv(const pyib_mesh *m, const void *face, void *cell, void *stream);          /* IBCore.cdiv */
+/* IBCore.flow_mat + DirBC (ib_core_np.py:213-220, ib_driver_np.py:387-388):
+ * w = rot(integr(V(interp(curl p)))), V x = rou x - hdt*vol_ci*cdiv(AoL*visc*cgrad0 x),
+ * hdt = 0.5*dt; w is zeroed on dir_mask when mask_dir != 0.  If dot_out is
+ * non-NULL, *dot_out (device) receives p.w in double (deterministic). */
+int pyib_flow_mat(const pyib_mesh *m, const pyib_work *ws, double rou, double visc, double hdt,
+                  const void *p, void *w, int mask_dir, double *dot_out, void *stream);
+
+/* ---- time-step pieces (ib_driver_np.py:267-303, 458-496) ---- */
+/* Driver.calc_vel: s_out = s with inlet values; U = curl(s_out); vc (ncell+nghost) x 3 */
+int pyib_calc_vel(const pyib_mesh *m, const void *s_in, void *s_out, void *U, void *vc,
+                  const double *lid_vec, void *stream);
+/* Driver.conv_diff + momentum residual:
+ * out = [base - a_vc*vc | a_vc*vc] - k_conv*conv + k_diff*diff (+ k_force*force)
+ * base may be NULL (then the first term is +a_vc*vc); force may be NULL. */
+int pyib_momentum_residual(const pyib_mesh *m, const void *U, const void *vc, double visc,
+                           const void *base, double a_vc, double k_conv, double k_diff,
+                           const void *force, double k_force, void *out, void *stream);
+/* rot(integr_c2f(res_c)) (ib_driver_np.py:495-496) */
+int pyib_curl_rhs(const pyib_mesh *m, const pyib_work *ws, const void *res_c, void *res_e, void *stream);
+
+/* ---- Jacobi-PCG of Driver.helm_solve (ib_driver_np.py:311-410) ---- */
+/* r = res_e (0 on dir), x = s, p = r*prec; scal[eta]=p.r, scal[l2_init]=r.r,
+ * scal[err_tot] = (x.x + 1e-16) tol1^2 + tol2^2 eta */
+int pyib_cg_begin(const pyib_mesh *m, const pyib_work *ws, const pyib_cg *cg, const void *res_e,
+                  const void *s, double tol1, double tol2, void *stream);
+/* n_iter PCG iterations.  The last iteration stops before the direction
+ * update when skip_last_direction != 0, so the host can test scal[eta_new]. */
+int pyib_cg_iterate(const pyib_mesh *m, const pyib_work *ws, const pyib_cg *cg, double rou,
+                    double visc, double hdt, int32_t n_iter, int32_t skip_last_direction, void *stream);
+/* p = r*prec + beta*p; eta = eta_new */
+int pyib_cg_direction(const pyib_mesh *m, const pyib_work *ws, const pyib_cg *cg, void *stream);
+/* true residual tr = res_e - A(x - s) (0 on dir); out2[0] = (tr*prec).tr, out2[1] = tr.tr */
+int pyib_cg_true_residual(const pyib_mesh *m, const pyib_work *ws, const pyib_cg *cg,
+                          const void *res_e, const void *s, void *tmp_edge, void *tr,
+                          double rou, double visc, double hdt, double *out2, void *stream);
+
+/* Driver.step guess: out = s + (s - s_old)*dt/dt_old (ib_driver_np.py:461), float64 */
+int pyib_extrapolate(int64_t n, const double *s, const double *s_old, double dt, double dt_old,
+                     double *out, void *stream);
+
+/* ---- the same CG on the structured layout (ib_driver_np.py:311-410) ----
+ * Workspace cell_a/cell_b hold 3*ncell (internal) values; partials >= 3*PYIB_MAX_PARTIALS. */
+int pyib_grid_flow_mat(const pyib_grid *g, const pyib_work *ws, double rou, double visc, double hdt,
+                       const void *p, void *w, double *dot_out, void *stream);
+int pyib_grid_cg_begin(const pyib_grid *g, const pyib_work *ws, const pyib_cg *cg, const void *res,
+                       const void *s, double tol1, double tol2, void *stream);
+int pyib_grid_cg_iterate(const pyib_grid *g, const pyib_work *ws, const pyib_cg *cg, double rou,
+                         double visc, double hdt, int32_t n_iter, int32_t skip_last_direction,
+                         void *stream);
+int pyib_grid_cg_direction(const pyib_grid *g, const pyib_cg *cg, void *stream);
+int pyib_grid_cg_true_residual(const pyib_grid *g, const pyib_work *ws, const pyib_cg *cg,
+                               const void *res, const void *s, void *tmp, void *tr, double rou,
+                               double visc, double hdt, double *out2, void *stream);
+/* out[i] = idx[i] >= 0 ? sigma[i]*in[idx[i]] : 0 (renumbering between PyIB's and the layout's order) */
+int pyib_permute(int dtype_in, const void *in, int dtype_out, void *out, int64_t n_out,
+                 const int64_t *idx, const int8_t *sigma, void *stream);
+
+/* ---- mixed precision helpers ---- */
+int pyib_convert(int dtype_in, const void *in, int dtype_out, void *out, int64_t n, void *stream);
+/* y[i] += a[i] (a in dtype_a, y double) */
+int pyib_add_into_f64(int dtype_a, const void *a, double *y, int64_t n, void *stream);
+
+/* ---- generic vector helpers (double reductions) ---- */
+int pyib_dot(int dtype, const void *a, const void *b, const void *c, int64_t n, double *partials,
+             double *out, void *stream); /* sum a*b (*c if c) */
+int pyib_axpby(int dtype, double a, const void *x, double b, const void *y, void *out, int64_t n,
+               void *stream);
+
+/* ---- immersed boundary (ib_immersed.py:12-120) ---- */
+/* Roma-Peskin 3-point stencil of every marker: 27 taps (the 125-tap window of
+ * the reference has zero weight outside the 3^3 core).  cell[m*27+t] = finest
+ * cell id or -1 when outside the finest box; w[...] = delta product (double). */
+int pyib_ib_stencil(int64_t nmark, const double *markers, double pos_ref_x, double pos_ref_y,
+                    double pos_ref_z, double dx, double hh, int32_t nx, int32_t ny, int32_t nz,
+                    int32_t *cell, double *w, void *stream);
+/* vel[m] = sum_t vc[cell]*w*hh^3 (ImmersedBoundary._interp_stencil) */
+int pyib_ib_interp(int dtype, int64_t nmark, const int32_t *cell, const double *w, double hh3,
+                   const void *vc, double *vel, void *stream);
+/* Deterministic spreading (ImmersedBoundary._spread_stencil): the (cell, tap)
+ * pairs are radix-sorted by cell (stable), then every cell's contributions are
+ * summed in marker order, as NumPy's bincount does.  keys/vals are caller
+ * workspaces of nmark*27 entries; temp is the radix-sort scratch of
+ * pyib_ib_sort_temp_bytes bytes.  pyib_ib_spread writes scale*sum into the
+ * touched cells of out (ncell x 3), or adds it when accumulate != 0; cells no
+ * marker touches are left as they are (the caller clears out first when it
+ * wants the reference's fresh array). */
+int pyib_ib_sort_temp_bytes(int64_t n, int64_t *bytes);
+int pyib_ib_sort(int64_t nmark, const int32_t *cell, int32_t *keys_tmp, int32_t *vals_tmp,
+                 int32_t *keys_sorted, int32_t *vals_sorted, void *temp, int64_t temp_bytes,
+                 void *stream);
+int pyib_ib_spread(int dtype, int64_t nmark, const int32_t *keys_sorted, const int32_t *vals_sorted,
+                   const double *w, const double *flag, const double *ds3, double scale,
+                   int accumulate, void *out, void *stream);
+/* f_L = (vel_desired - interp)/dt (ImmersedBoundary.direct_force) */
+int pyib_ib_force(int64_t nmark, const double *vel_desired, const double *vel_ib, double dt,
+                  double *f_lag, void *stream);
+
+/* ---- FP64 reductions of the closed-body load correction (ib_interior.py:57-70, 291-296) ---- */
+/* over candidate cells idx (pos/vol are per candidate): sums of v*wv, (pos x v)*wv
+ * and wv with wv the smoothed-sphere indicator times volume (or fixed_wv when
+ * fixed != 0); out7 = [lin(3), ang(3), vol], density applied by the caller */
+int pyib_interior_momenta(int dtype, int64_t n, const int32_t *idx, const double *pos, const double *vol,
+                          const void *vc, double cx, double cy, double cz, double radius, double width,
+                          double rho, int fixed, const double *fixed_wv, double *partials, double *out7,
+                          void *stream);
+/* out6 = [sum F*vol (3), sum (pos x F)*vol (3)] over force-support cells idx */
+int pyib_fluid_load(int dtype, int64_t n, const int32_t *idx, const double *pos, const double *vol,
+                    const void *F, double *partials, double *out6, void *stream);
+/* diagnostics (ib_case.py:1147-1151): out2[0] = max |vol_ci*cdiv(U)|, out2[1] = max |vc|
+ * over volume cells; tmp_cell holds >= ncell + 2*PYIB_MAX_PARTIALS doubles */
+int pyib_diag_max(const pyib_mesh *m, const void *U, const void *vc, void *tmp_cell, double *out2,
+                  void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYIB_HIP_H */

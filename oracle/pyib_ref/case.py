@@ -130,7 +130,7 @@ def uses_cavity(config):
     return kind == "sphere_concentric" or (kind == "nested_box" and float(config["physics"]["inflow_speed"]) == 0.0)
 
 
-def build_runtime(config, mesh=None, sparse=None):
+def build_runtime(config, mesh=None, sparse=None, geometry=None):
     """ib_case.py:633-809 (explicit IB only)."""
     ibc = config["ib"]
     radius = float(ibc["radius"])
@@ -140,7 +140,7 @@ def build_runtime(config, mesh=None, sparse=None):
     else:
         dx = 2.0 * radius / int(config["mesh"]["cells_per_diameter"]) if config["mesh"]["kind"] != "uniform_box" \
             else float(config["mesh"]["dx"])
-    core = Core(mesh)
+    core = Core(mesh, geometry=geometry)
     ph = config["physics"]
     U = float(ph["inflow_speed"])
     Re = float(ph["reynolds"])
@@ -179,7 +179,7 @@ def build_runtime(config, mesh=None, sparse=None):
     load = ClosedSphereLoadCorrection(core.cpos[:core.ncell], 1.0 / core.vol_ci, reference_center=centre,
                                       maximum_translation=mt, radius=radius, smoothing_width=dx, density=rho,
                                       force_support_width=1.5 * ib.hh)
-    fref = 0.5 * rho * uref ** 2 * np.pi * radius ** 2
+    fref = 0.5 * rho * uref ** 2 * (np.pi * radius ** 2)
     return Runtime(config, mesh, core, drv, ib, load, markers, widths, fref, uref, dx, dt,
                    int(config["time"]["steps"]), int(config["time"]["inner_iterations"]))
 

@@ -23,8 +23,16 @@ def _avg_to_cells(F2C, m1, ncell, fvals):
 class Core:
     """ib_core_np.py:68-171 (IBCore) in float64."""
 
-    def __init__(self, mesh, ND=3):
+    def __init__(self, mesh, ND=3, geometry=None):
         self.ND = ND
+        if geometry is not None:
+            # Geometry already computed by an equivalent builder (bitwise-equal, checked by the
+            # tests): reuse it so that a timing run does not pay the set-up twice.
+            for name in ("F2C", "ncell", "nface", "nedge", "m1", "lf_face", "lf_edge", "epos", "te",
+                         "fe_sign_link", "nf", "Af", "fpos", "cpos", "rf1", "rf2", "rf", "vol_ci", "AoL"):
+                setattr(self, name, getattr(geometry, name))
+            self.E2N = mesh.edges_to_nodes
+            return
         nodes = mesh.nodes
         self.E2N = E2N = mesh.edges_to_nodes
         self.F2C = F2C = mesh.faces_to_cells

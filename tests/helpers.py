@@ -1,0 +1,47 @@
+"""Shared test helpers: configs of the parity cases and array comparisons."""
+import json
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def golden(name):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+def golden_config(name):
+    with golden(f"traj_{name}.npz") as g:
+        return json.loads(str(g["config"]))
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    d = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (d if d > 0 else 1.0))
+
+
+def small_case(kind="oscillating", device="hip", precision="float64", steps=3, mesh=None, **over):
+    """Small parity cases (normalised by the product's validate_config)."""
+    raw = {"name": f"parity_{kind}", "runtime": {"device": device, "precision": precision},
+           "time": {"dt": 0.01, "steps": steps, "inner_iterations": 4},
+           "solver": {"sparse": False, "cg_max_iterations": 2000}}
+    if kind == "oscillating":
+        raw["case"] = {"kind": "oscillating_sphere"}
+        raw["ib"] = {"marker_spacing": 0.125, "motion": {"kind": "oscillation", "axis": [1, 0, 0],
+                                                          "amplitude": 0.05, "frequency": 0.5729577951308232}}
+        raw["physics"] = {"reynolds": 0.2, "inflow_speed": 0.0}
+        raw["mesh"] = mesh or {"kind": "nested_box", "cells_per_diameter": 8, "half_extents_d": [1.5, 1.25, 1.25],
+                               "levels": 2}
+    else:
+        raw["case"] = {"kind": "fixed_sphere"}
+        raw["ib"] = {"marker_spacing": 0.0625}
+        raw["physics"] = {"reynolds": 100.0, "inflow_speed": 1.0}
+        raw["mesh"] = mesh or {"kind": "nested_box", "cells_per_diameter": 8, "half_extents_d": [1.5, 1.25, 1.25],
+                               "levels": 2}
+    for k, v in over.items():
+        raw.setdefault(k, {}).update(v)
+    import pyib_hip.case as HC
+    return HC.validate_config(raw)

@@ -1,0 +1,142 @@
+"""Run-level behaviour on the GPU: IB transfer, loads, resume, strict CG, mixed precision.
+
+Tolerances: IB interpolation/spreading 1e-15 relative vs the oracle and
+bitwise run-to-run; load reductions 1e-13; resume bitwise identical to an
+uninterrupted run; mixed precision vs float64 <= 1e-6 relative on the forces.
+"""
+import copy
+import json
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+import pyib_hip  # noqa: E402,F401
+import pyib_hip.case as HC  # noqa: E402
+import pyib_hip.mesh as PM  # noqa: E402
+from pyib_hip.immersed import HipImmersedBoundary  # noqa: E402
+from pyib_hip.interior import HipLoadCorrection  # noqa: E402
+from oracle.pyib_ref import immersed as OI  # noqa: E402
+from oracle.pyib_ref import interior as OIn  # noqa: E402
+from oracle.pyib_ref import core as OCore  # noqa: E402
+from tests.helpers import ROOT, rel, small_case  # noqa: E402
+
+
+def T(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def test_ib_transfer_matches_oracle_and_is_deterministic():
+    mesh = PM.make_nested_mesh(0.0625, PM.nested_box_boxes([1.0, 1.0, 1.0], 2))
+    oc = OCore.Core(mesh)
+    rng = np.random.default_rng(5)
+    mk = rng.uniform(-0.4, 0.4, size=(300, 3))
+    ds = rng.uniform(0.03, 0.07, 300)
+    f = rng.standard_normal((300, 3))
+    vc = rng.standard_normal((oc.ncell, 3))
+    oib = OI.ImmersedBoundary(mesh)
+    hib = HipImmersedBoundary(mesh)
+    hib.stencil(T(mk))
+    vel = hib.interp(T(vc)).cpu().numpy()
+    assert rel(vel, oib.interp(vc, mk)) < 1e-15
+    out1 = torch.zeros((oc.ncell, 3), dtype=torch.float64, device="cuda")
+    hib.spread_into(out1, T(f), T(ds ** 3))
+    ref = oib.spread(f, mk, ds)
+    assert rel(out1.cpu().numpy(), ref) < 1e-15
+    out2 = torch.zeros_like(out1)
+    hib.stencil(T(mk))
+    hib.spread_into(out2, T(f), T(ds ** 3))
+    assert torch.equal(out1, out2)
+    # accumulate mode: out += 2*spread
+    hib.spread_into(out2, T(f), T(ds ** 3), scale=2.0, accumulate=True)
+    assert rel(out2.cpu().numpy(), 3.0 * ref) < 1e-15
+
+
+def test_load_reductions_match_oracle():
+    mesh = PM.make_nested_mesh(0.0625, PM.nested_box_boxes([1.0, 1.0, 1.0], 2))
+    oc = OCore.Core(mesh)
+    rng = np.random.default_rng(6)
+    pos, vol = oc.cpos, 1.0 / oc.vol_ci
+    kw = dict(reference_center=[0, 0, 0], maximum_translation=[0.05, 0, 0], radius=0.25, smoothing_width=0.0625,
+              density=1.3, force_support_width=1.5 * 0.0625)
+    o = OIn.ClosedSphereLoadCorrection(pos, vol, **kw)
+    h = HipLoadCorrection(pos, vol, **kw)
+    v0, v1 = rng.standard_normal((oc.ncell, 3)), rng.standard_normal((oc.ncell, 3))
+    F = rng.standard_normal((oc.ncell, 3))
+    o.initialize(v0, [0.01, 0, 0])
+    h.initialize(T(v0), [0.01, 0, 0])
+    lo = o.advance(v1, [0.02, 0, 0], 0.01, F)
+    lh = h.advance(T(v1), [0.02, 0, 0], 0.01, T(F))
+    for name in ("force_fluid", "moment_fluid", "body_force", "body_moment"):
+        assert rel(getattr(lh, name), getattr(lo, name)) < 1e-12, name
+    assert abs(float(lh.interior_momenta.indicator_volume) - float(lo.interior_momenta.indicator_volume)) < 1e-13
+
+
+def _cfg(tmp_steps, **over):
+    cfg = small_case("oscillating", steps=tmp_steps)
+    cfg["output"] = {"sample_every": 1, "checkpoint_every": 2, "field_every": 0, "field_start_step": 0,
+                     "save_field": True}
+    for k, v in over.items():
+        cfg[k].update(v)
+    return HC.validate_config(cfg)
+
+
+def test_resume_is_bitwise_identical(tmp_path):
+    full = HC.run_case(_cfg(4), tmp_path / "a", quiet=True)
+    part = HC.run_case(_cfg(2), tmp_path / "b", quiet=True)
+    assert part.completed_step == 2
+    res = HC.run_case(_cfg(4), tmp_path / "b", resume=tmp_path / "b" / "checkpoint.npz", quiet=True)
+    assert res.start_step == 2 and res.completed_step == 4
+    assert np.array_equal(full.s, res.s) and np.array_equal(full.ib_force_E, res.ib_force_E)
+    ha = [{k: v for k, v in r.items() if k != "milliseconds_per_step"} for r in full.history]
+    hb = [{k: v for k, v in r.items() if k != "milliseconds_per_step"} for r in res.history]
+    assert ha == hb
+    with np.load(tmp_path / "a" / "final_field.npz", allow_pickle=False) as ff:
+        assert str(ff["device"]) == "hip" and ff["vc"].shape[1] == 3
+    md = json.loads((tmp_path / "a" / "metadata.json").read_text())
+    assert md["complete"] and md["backend"] == "hip/pyib_hip"
+
+
+def test_resume_rejects_foreign_checkpoint(tmp_path):
+    HC.run_case(_cfg(2), tmp_path / "a", quiet=True)
+    other = _cfg(4, physics={"reynolds": 0.3})
+    with pytest.raises(HC.CaseConfigError, match="configuration"):
+        HC.run_case(other, tmp_path / "a", resume=tmp_path / "a" / "checkpoint.npz", quiet=True)
+
+
+def test_strict_cg_failure_exit_code(tmp_path):
+    raw = copy.deepcopy(small_case("oscillating", steps=1))
+    raw["solver"]["cg_max_iterations"] = 3
+    raw["solver"]["fail_on_cg_nonconvergence"] = True
+    case = tmp_path / "c.json"
+    case.write_text(json.dumps(raw))
+    r = subprocess.run([sys.executable, "-m", "pyib_hip.cli", str(case), "--outdir", str(tmp_path / "o"), "--quiet"],
+                       cwd=ROOT, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2 and "did not converge" in r.stderr
+
+
+def _forces(cfg, steps):
+    rt = HC.build_runtime(cfg)
+    s, s_old, vc = HC.initial_state(rt)
+    out = []
+    for step in range(1, steps + 1):
+        rt.driver.time = (step - 1) * rt.dt
+        s, vc, s_old = rt.driver.step(s, s_old, rt.dt, rt.dt, inner=rt.inner_iterations)
+        rt.driver.time = step * rt.dt
+        out.append(np.asarray(rt.load_correction.advance(vc, HC.sphere_center(rt.config, rt.driver.time), rt.dt,
+                                                         rt.driver.ib_force_E).body_force))
+    return np.array(out), s.cpu().numpy(), rt
+
+
+def test_mixed_precision_force_error():
+    f64, s64, _ = _forces(small_case("oscillating", steps=4), 4)
+    fmx, smx, rt = _forces(small_case("oscillating", precision="mixed", steps=4), 4)
+    err = np.abs(fmx - f64).max() / np.abs(f64).max()
+    print(f"[gpu] mixed vs float64: force rel err {err:.2e}, s rel err {rel(smx, s64):.2e}")
+    assert err <= 1e-6
+    assert rel(smx, s64) <= 1e-6
+    assert rt.driver.helmholtz_converged_last_step
