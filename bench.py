@@ -178,6 +178,47 @@ def cpu_baseline(mesh, geo, cfg, cg_iters_per_step, seconds, log):
             "s_per_pcg_iteration": t_iter, "s_per_step": t_step}
 
 
+def kernels_only(n_iter, precision, log):
+    """N PCG iterations of the headline operator on a seeded right-hand side, then a plain
+    copy kernel of 2**26 doubles (read 512 MiB, write 512 MiB) to calibrate HBM counters."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    import pyib_hip  # noqa: F401
+    from pyib_hip import _native as N
+    from pyib_hip import case as HC
+    from pyib_hip.core import dev_stream, ptr
+
+    cfg = HC.validate_config(headline_config(1, precision))
+    mesh, _, _ = HC.build_mesh(cfg)
+    rt = HC.build_runtime(cfg, mesh=mesh)
+    drv, g = rt.driver, rt.driver.grid
+    log(f"[kernels] {g.nedge:,} internal edges ({rt.core.nedge:,} mesh edges)")
+    rng = np.random.default_rng(0)
+    res = rng.standard_normal(rt.core.nedge)
+    res[drv.dir_mask] = 0.0
+    res_d = rt.core.tensor(res)
+    s0 = torch.zeros(rt.core.nedge, dtype=torch.float64, device="cuda")
+    prec = "float32" if precision == "mixed" else "float64"
+    m, ws = g.struct(prec), g.work_struct(prec)
+    cg = drv._cgi32 if prec == "float32" else drv._cgi
+    dt = torch.float32 if prec == "float32" else torch.float64
+    r_i = g.to_internal(res_d, dtype=dt)
+    z_i = torch.zeros(g.nedge, dtype=dt, device="cuda")
+    N.call("pyib_grid_cg_begin", m, ws, C.byref(cg), ptr(r_i), ptr(z_i), 0.0, 1e-30, dev_stream())
+    N.call("pyib_grid_cg_iterate", m, ws, C.byref(cg), float(drv.rou), float(drv.visc), 0.5 * rt.dt, int(n_iter), 0,
+           dev_stream())
+    torch.cuda.synchronize()
+    a = torch.ones(1 << 26, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    N.call("pyib_convert", N.F64, ptr(a), N.F64, ptr(b), C.c_int64(a.numel()), dev_stream())
+    torch.cuda.synchronize()
+    log(f"[kernels] ran {n_iter} {prec} PCG iterations and one 2^26-double copy")
+    print(json.dumps({"kernels_only": n_iter, "precision": prec, "nedge_internal": g.nedge,
+                      "nedge": rt.core.nedge, "ncell_internal": g.ncell}), flush=True)
+    return 0
+
+
 def main(argv=None):
     p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     p.add_argument("--gpus", type=int, default=1)
@@ -186,12 +227,15 @@ def main(argv=None):
     p.add_argument("--precision", choices=("mixed", "float64", "both"), default="float64")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=20.0)
+    p.add_argument("--kernels-only", type=int, default=0, help="profiling: N PCG iterations + a copy kernel")
     a = p.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus})",
               file=sys.stderr)
         return 2
     log = lambda msg: print(msg, file=sys.stderr, flush=True)
+    if a.kernels_only:
+        return kernels_only(a.kernels_only, "mixed" if a.precision == "mixed" else "float64", log)
     precs = ["float64", "mixed"] if a.precision == "both" else [a.precision]
     results = {}
     for prec in precs:

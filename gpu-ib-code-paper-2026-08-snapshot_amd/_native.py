@@ -104,6 +104,17 @@ _SIGS = {
                                         C.c_double, C.c_double, C.c_double, C.c_int, vp, vp, vp, vp]),
     "pyib_fluid_load": (C.c_int, [C.c_int, C.c_int64, vp, vp, vp, vp, vp, vp, vp]),
     "pyib_diag_max": (C.c_int, [C.POINTER(Mesh), vp, vp, vp, vp, vp]),
+    "pyib_calc_vel_adjoint": (C.c_int, [C.POINTER(Mesh), vp, vp, vp, vp, vp, vp]),
+    "pyib_momentum_residual_adjoint": (C.c_int, [C.POINTER(Mesh), vp, vp, vp, C.c_double, C.c_int, C.c_double,
+                                                 C.c_double, C.c_double, vp, vp, vp, vp]),
+    "pyib_ib_dweights": (C.c_int, [C.c_int64, vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.c_int32, C.c_int32, C.c_int32, vp, vp]),
+    "pyib_ib_marker_grad": (C.c_int, [C.c_int64, vp, vp, vp, vp, vp, C.c_double, vp, vp]),
+    "pyib_ib_gather": (C.c_int, [C.c_int64, vp, vp, vp, vp, vp, vp]),
+    "pyib_fluid_load_adjoint": (C.c_int, [C.c_int64, vp, vp, C.c_double, C.c_double, C.c_double, vp, vp]),
+    "pyib_interior_momentum_adjoint": (C.c_int, [C.c_int64, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
+                                                 C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                                 C.c_double, vp, vp, vp, vp]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -177,6 +177,38 @@ int pyib_grid_cg_true_residual(const pyib_grid *g, const pyib_work *ws, const py
 int pyib_permute(int dtype_in, const void *in, int dtype_out, void *out, int64_t n_out,
                  const int64_t *idx, const int8_t *sigma, void *stream);
 
+/* ---- adjoints (transposes) of the time-step operators, float64 ----
+ * face_ghost: nface, ghost index k of boundary face f or -1. */
+/* Driver.calc_vel^T: folds the ghost rows of g_vc into their owners (g_vc is
+ * modified), then g_U = interp^T g_vc + ghost terms + g_U_in, g_s = curl^T g_U
+ * with the inlet edges zeroed. */
+int pyib_calc_vel_adjoint(const pyib_mesh *m, const int32_t *face_ghost, double *g_vc,
+                          const double *g_U_in, double *g_U, double *g_s, void *stream);
+/* momentum residual^T: g_vc (ncell+nghost, 3) += d out/d vc^T g_out, g_U += d out/d U^T g_out */
+int pyib_momentum_residual_adjoint(const pyib_mesh *m, const int32_t *face_ghost, const double *U,
+                                   const double *vc, double visc, int has_base, double a_vc,
+                                   double k_conv, double k_diff, const double *g_out, double *g_vc,
+                                   double *g_U, void *stream);
+/* d w / d marker for every (marker, tap) of pyib_ib_stencil: dw (nmark*27 x 3) */
+int pyib_ib_dweights(int64_t nmark, const double *markers, double pos_ref_x, double pos_ref_y,
+                     double pos_ref_z, double dx, double hh, int32_t nx, int32_t ny, int32_t nz,
+                     double *dw, void *stream);
+/* out[m] += scale[m]*scale_all*sum_t (field[cell].vec[m]) dw[m,t]  (scale may be NULL) */
+int pyib_ib_marker_grad(int64_t nmark, const int32_t *cell, const double *dw, const double *field,
+                        const double *vec, const double *scale, double scale_all, double *out,
+                        void *stream);
+/* out[m] = scale[m] * sum_t field[cell] w[m,t]  (spread^T) */
+int pyib_ib_gather(int64_t nmark, const int32_t *cell, const double *w, const double *field,
+                   const double *scale, double *out, void *stream);
+/* g_F[idx] += g * vol  (force_fluid^T) */
+int pyib_fluid_load_adjoint(int64_t n, const int32_t *idx, const double *vol, double g0, double g1,
+                            double g2, double *gF, void *stream);
+/* linear interior momentum^T: g_vc[idx] += rho g wv; gcenter3 = d(P.g)/d center */
+int pyib_interior_momentum_adjoint(int64_t n, const int32_t *idx, const double *pos, const double *vol,
+                                   const double *vc, double cx, double cy, double cz, double radius,
+                                   double width, double rho, double g0, double g1, double g2,
+                                   double *gvc, double *partials, double *gcenter3, void *stream);
+
 /* ---- mixed precision helpers ---- */
 int pyib_convert(int dtype_in, const void *in, int dtype_out, void *out, int64_t n, void *stream);
 /* y[i] += a[i] (a in dtype_a, y double) */

@@ -45,3 +45,27 @@ def small_case(kind="oscillating", device="hip", precision="float64", steps=3, m
         raw.setdefault(k, {}).update(v)
     import pyib_hip.case as HC
     return HC.validate_config(raw)
+
+
+def parity_tol(floor, base=1e-12, factor=10.0):
+    """Tolerance of a float64 parity check: the reference's own CSR (solver.sparse) and
+    matrix-free flow_mat paths differ by `floor` on the same case; a HIP result must stay
+    within max(base, factor*floor) of the matrix-free oracle."""
+    return max(base, factor * floor)
+
+
+FORCE_COLUMNS = ("body_force_x", "body_force_y", "body_force_z")
+
+
+def force_history(rows, cols=FORCE_COLUMNS):
+    """(steps, 3) body-force history from history rows (dicts) or a (history, columns) pair."""
+    if isinstance(rows, tuple):
+        hist, names = rows
+        return np.stack([hist[:, list(names).index(c)] for c in cols], axis=1)
+    return np.array([[r[c] for c in cols] for r in rows])
+
+
+def force_rel(ours, ref):
+    """Relative L2 of a force history: ||F - F_ref|| / ||F_ref|| over every step and component (the
+    cross-stream components of a symmetric case are near zero, so each component is not scaled by itself)."""
+    return rel(np.asarray(ours), np.asarray(ref))

@@ -133,10 +133,55 @@ def _forces(cfg, steps):
 
 
 def test_mixed_precision_force_error():
-    f64, s64, _ = _forces(small_case("oscillating", steps=4), 4)
-    fmx, smx, rt = _forces(small_case("oscillating", precision="mixed", steps=4), 4)
+    """Mixed precision vs float64: force history to 1e-6 relative; the face-flux field error is
+    stated and bounded by 1e-5 (both solvers stop at the same 1e-10/1e-9 mixed tolerance, so the
+    two solutions differ by the solver tolerance times the operator's conditioning, not by float32
+    round-off).  The stream function itself also carries a gauge (gradient) component that the
+    equations leave free; it is printed for reference only."""
+    f64, s64, r64 = _forces(small_case("oscillating", steps=4), 4)
+    fmx, smx, rmx = _forces(small_case("oscillating", precision="mixed", steps=4), 4)
     err = np.abs(fmx - f64).max() / np.abs(f64).max()
-    print(f"[gpu] mixed vs float64: force rel err {err:.2e}, s rel err {rel(smx, s64):.2e}")
+    U64 = r64.core.curl(torch.from_numpy(s64).cuda()).cpu().numpy()
+    Umx = rmx.core.curl(torch.from_numpy(smx).cuda()).cpu().numpy()
+    print(f"[gpu] mixed vs float64: force rel err {err:.2e}, face flux rel err {rel(Umx, U64):.2e}, "
+          f"stream function (incl. gauge) {rel(smx, s64):.2e}")
     assert err <= 1e-6
-    assert rel(smx, s64) <= 1e-6
-    assert rt.driver.helmholtz_converged_last_step
+    assert rel(Umx, U64) <= 1e-5
+    assert rmx.driver.helmholtz_converged_last_step
+
+
+def test_reference_d32_run_history():
+    """The reference's own D32 oscillating-sphere run (5-level concentric mesh, CUDA FP64,
+    checks/pressure-validation-20260925.zip runs/D32): the first sampled steps of its
+    history.csv against the HIP path on the same case file."""
+    import csv
+    import os
+    from tests.helpers import GOLDEN
+    ref = json.load(open(os.path.join(GOLDEN, "reference_runs.json")))["D32"]["case"]
+    rows = list(csv.DictReader(open(os.path.join(GOLDEN, "reference_d32_history_head.csv"))))
+    want = [r for r in rows if int(r["step"]) <= 10]
+    cfg = copy.deepcopy(ref)
+    cfg["runtime"]["device"] = "hip"
+    cfg["time"]["steps"] = 10
+    cfg = HC.validate_config(cfg)
+    print("[gpu] reference D32 case, 10 steps", flush=True)
+    rt = HC.build_runtime(cfg)
+    s, s_old, vc = HC.initial_state(rt)
+    got = {}
+    for step in range(1, 11):
+        rt.driver.time = (step - 1) * rt.dt
+        s, vc, s_old = rt.driver.step(s, s_old, rt.dt, rt.dt, inner=rt.inner_iterations)
+        rt.driver.time = step * rt.dt
+        rt.load_correction.advance(vc, HC.sphere_center(rt.config, rt.driver.time), rt.dt, rt.driver.ib_force_E)
+        got[step] = HC.diagnostics(rt, step, s, vc, 0.0)
+    worst = {}
+    for r in want:
+        g = got[int(r["step"])]
+        print(f"[gpu] step {r['step']}: CG {g['helmholtz_iterations']} vs {r['helmholtz_iterations']}, "
+              f"Fx {g['body_force_x']:.15e} vs {float(r['body_force_x']):.15e}")
+        assert g["helmholtz_iterations"] == int(r["helmholtz_iterations"])
+        for col in ("body_force_x", "force_fluid_x", "interior_momentum_x", "slip_max", "speed_max"):
+            a, b = g[col], float(r[col])
+            worst[col] = max(worst.get(col, 0.0), abs(a - b) / max(abs(b), 1e-300))
+    print("[gpu] worst relative differences vs the reference's CUDA run:", worst)
+    assert worst["body_force_x"] < 1e-8 and worst["force_fluid_x"] < 1e-8
