@@ -11,7 +11,7 @@ import os
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpyib_hip.so")
 
 F64, F32 = 0, 1
-MAX_PARTIALS = 2048
+MAX_PARTIALS = 65536
 NSCAL = 8
 S_ETA, S_ETA_NEW, S_DEN, S_ALPHA, S_BETA, S_RR, S_ERR, S_L2I = range(8)
 
@@ -43,9 +43,9 @@ class Grid(C.Structure):
     _fields_ = [
         ("dtype", C.c_int32), ("nlevels", C.c_int32), ("ncell", C.c_int64), ("nedge", C.c_int64),
         ("lev", Level * MAX_LEVELS), ("cell_reg", vp), ("edge_reg", vp), ("dir", vp),
-        ("g1_nrows", C.c_int64), ("g1_row", vp), ("g1_ptr", vp), ("g1_col", vp), ("g1_coef", vp),
-        ("v_nrows", C.c_int64), ("v_row", vp), ("v_ptr", vp), ("v_col", vp), ("v_coef", vp),
-        ("g3_nrows", C.c_int64), ("g3_row", vp), ("g3_ptr", vp), ("g3_col", vp), ("g3_coef", vp),
+        ("g1_nslice", C.c_int64), ("g1_row", vp), ("g1_width", vp), ("g1_sptr", vp), ("g1_col", vp), ("g1_coef", vp),
+        ("v_nslice", C.c_int64), ("v_row", vp), ("v_width", vp), ("v_sptr", vp), ("v_col", vp), ("v_coef", vp),
+        ("g3_nslice", C.c_int64), ("g3_row", vp), ("g3_width", vp), ("g3_sptr", vp), ("g3_col", vp), ("g3_coef", vp),
     ]
 
 
@@ -105,6 +105,7 @@ _SIGS = {
     "pyib_fluid_load": (C.c_int, [C.c_int, C.c_int64, vp, vp, vp, vp, vp, vp, vp]),
     "pyib_diag_max": (C.c_int, [C.POINTER(Mesh), vp, vp, vp, vp, vp]),
     "pyib_calc_vel_adjoint": (C.c_int, [C.POINTER(Mesh), vp, vp, vp, vp, vp, vp]),
+    "pyib_curl_rhs_adjoint": (C.c_int, [C.POINTER(Mesh), C.POINTER(Work), vp, vp, vp]),
     "pyib_momentum_residual_adjoint": (C.c_int, [C.POINTER(Mesh), vp, vp, vp, C.c_double, C.c_int, C.c_double,
                                                  C.c_double, C.c_double, vp, vp, vp, vp]),
     "pyib_ib_dweights": (C.c_int, [C.c_int64, vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,

@@ -452,8 +452,23 @@ def _mesh_params(config):
     return {k: mc.get(k) for k in ("kind", "cells_per_diameter", "half_extents_d", "levels", "dx", "bounds")}
 
 
+_SETUP_MEMO = {}
+
+
 def setup_objects(config, mesh):
-    """Geometry, gather tables and structured layout of a mesh (cached when PYIB_HIP_SETUP_CACHE is set)."""
+    """Geometry, gather tables and structured layout of a mesh (cached on disk when
+    PYIB_HIP_SETUP_CACHE is set, and in this process for the mesh object last seen)."""
+    key = (id(mesh), json.dumps(_mesh_params(config), sort_keys=True))
+    hit = _SETUP_MEMO.get(key)
+    if hit is not None and hit[0] is mesh:
+        return hit[1]
+    out = _setup_objects(config, mesh)
+    _SETUP_MEMO.clear()
+    _SETUP_MEMO[key] = (mesh, out)
+    return out
+
+
+def _setup_objects(config, mesh):
     from . import setup_cache
     from .geometry import Geometry
     from .layout import Composite, Level, StructuredLayout, build_layout
@@ -463,7 +478,8 @@ def setup_objects(config, mesh):
         params["fingerprint"] = hashlib.sha256(np.ascontiguousarray(mesh.nodes).tobytes()).hexdigest()
     geo = setup_cache.cached("geometry", params, lambda: Geometry(mesh), Geometry)
     tables = setup_cache.cached("tables", params, lambda: build_tables(geo), Tables)
-    lay = setup_cache.cached("layout", params, lambda: build_layout(mesh, geo, tables), StructuredLayout,
+    lay = setup_cache.cached("layout", dict(params, layout_rules="margin1-geometric"),
+                             lambda: build_layout(mesh, geo, tables), StructuredLayout,
                              {"Composite": Composite, "Level": Level})
     return geo, tables, lay
 

@@ -68,6 +68,28 @@ __global__ void k_zero_idx(int64_t n, const int32_t* __restrict__ idx, double* _
     x[idx[i]] = 0.0;
 }
 
+// integr_c2f^T (ib_core_np.py:189-193 transposed): g_v[c] = sum_{faces with c = c1} g_F rf1
+// - sum_{faces with c = c2} g_F rf2, a cell-centric gather over the cell's face list.
+__global__ void k_integr_adj(int64_t ncell, const int32_t* __restrict__ ptr, const int32_t* __restrict__ mid,
+                             const int32_t* __restrict__ face, const double* __restrict__ rf1,
+                             const double* __restrict__ rf2, const double* __restrict__ gF, double* __restrict__ gv) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x) {
+    double a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+    const int j0 = ptr[c], jm = mid[c], j1 = ptr[c + 1];
+    for (int j = j0; j < jm; ++j) {
+      const int f = face[j];
+      const double u = gF[f];
+      a0 += u * rf1[3 * f]; a1 += u * rf1[3 * f + 1]; a2 += u * rf1[3 * f + 2];
+    }
+    for (int j = jm; j < j1; ++j) {
+      const int f = face[j];
+      const double u = gF[f];
+      b0 += u * rf2[3 * f]; b1 += u * rf2[3 * f + 1]; b2 += u * rf2[3 * f + 2];
+    }
+    gv[3 * c] = a0 - b0; gv[3 * c + 1] = a1 - b1; gv[3 * c + 2] = a2 - b2;
+  }
+}
+
 // ---- momentum residual^T ---------------------------------------------
 // out = [base - a*vc | a*vc] - kc*conv(U, vc) + kd*diff(vc) (+ kf*F); given g_out:
 // g_vc_ext += ..., g_U += ...   (cell, ghost and face passes)
@@ -231,7 +253,7 @@ __global__ void k_momentum_adj(int64_t n, const int32_t* __restrict__ idx, const
                                const double* __restrict__ vol, const double* __restrict__ vc, double cx, double cy,
                                double cz, double radius, double width, double rho, double g0, double g1, double g2,
                                double* __restrict__ gvc, double* __restrict__ partials) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[kFinBlock / 64];
   double acc[3] = {0, 0, 0};
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const double px = pos[3 * i] - cx, py = pos[3 * i + 1] - cy, pz = pos[3 * i + 2] - cz;
@@ -274,6 +296,16 @@ int pyib_calc_vel_adjoint(const pyib_mesh* m, const int32_t* face_ghost, double*
   if (rc) return rc;
   if (m->ninlet > 0) k_zero_idx<<<grid_for(m->ninlet), kBlock, 0, st>>>(m->ninlet, m->inlet_edge, g_s);
   return check_launch("pyib_calc_vel_adjoint");
+}
+
+int pyib_curl_rhs_adjoint(const pyib_mesh* m, const pyib_work* ws, const double* g_edge, double* g_res_c,
+                          void* stream) {
+  if (m->dtype != PYIB_F64) return fail_msg("pyib_curl_rhs_adjoint: float64 only");
+  int rc = pyib_curl(m, g_edge, ws->face, stream);     // rot^T = curl
+  if (rc) return rc;
+  k_integr_adj<<<grid_for(m->ncell), kBlock, 0, S(stream)>>>(m->ncell, m->cf_ptr, m->cf_mid, m->cf_face,
+      (const double*)m->rf1, (const double*)m->rf2, (const double*)ws->face, g_res_c);
+  return check_launch("pyib_curl_rhs_adjoint");
 }
 
 int pyib_momentum_residual_adjoint(const pyib_mesh* m, const int32_t* face_ghost, const double* U, const double* vc,
@@ -325,7 +357,7 @@ int pyib_interior_momentum_adjoint(int64_t n, const int32_t* idx, const double* 
   const int nb = red_grid_for(n);
   k_momentum_adj<<<nb, kBlock, 0, st>>>(n, idx, pos, vol, vc, cx, cy, cz, radius, width, rho, g0, g1, g2, gvc,
                                         partials);
-  k_finalize<<<1, kBlock, 0, st>>>(partials, nb, 3, gcenter3);
+  k_finalize<<<1, kFinBlock, 0, st>>>(partials, nb, 3, gcenter3);
   return check_launch("pyib_interior_momentum_adjoint");
 }
 

@@ -12,6 +12,8 @@ namespace pyib {
 constexpr int kBlock = 256;
 constexpr int kMaxGrid = 1 << 20;
 constexpr int kRedGrid = PYIB_MAX_PARTIALS;   // reduction blocks
+constexpr int kPerThread = 4;                 // elements per thread in the reducing kernels
+constexpr int kFinBlock = 1024;               // threads of the one-block finalize kernels
 enum { S_ETA = 0, S_ETA_NEW = 1, S_DEN = 2, S_ALPHA = 3, S_BETA = 4, S_RR = 5, S_ERR = 6, S_L2I = 7 };
 
 int fail(const char* where, hipError_t e);
@@ -27,9 +29,18 @@ inline int grid_for(int64_t n) {
   return static_cast<int>(g > kMaxGrid ? kMaxGrid : g);
 }
 inline int red_grid_for(int64_t n) {
-  int64_t g = (n + kBlock - 1) / kBlock;
+  int64_t g = (n + kBlock * kPerThread - 1) / (kBlock * kPerThread);
   if (g < 1) g = 1;
   return static_cast<int>(g > kRedGrid ? kRedGrid : g);
+}
+
+// Block index remapped so that each of the 8 XCDs (which receive workgroups
+// round-robin by blockIdx) works on one contiguous 1/8 of the index range: the
+// stencil neighbours of a block (+-1 row, +-1 plane) then sit in the same XCD's L2.
+__device__ __forceinline__ int64_t xcd_block() {
+  const int64_t nb = gridDim.x, b = blockIdx.x;
+  const int64_t q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+  return x * q + (x < r ? x : r) + i;
 }
 
 template <typename T>
@@ -38,6 +49,7 @@ __device__ __forceinline__ T sload(const T* __restrict__ x, int v) {
 }
 
 // Block-wide sum of one double (fixed tree => deterministic); valid in thread 0.
+// sh holds blockDim.x/64 <= kFinBlock/64 doubles.
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;

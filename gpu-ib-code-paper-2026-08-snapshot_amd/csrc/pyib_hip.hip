@@ -30,7 +30,7 @@ int fail_msg(const char* msg) {
 
 // Sum nsum rows of `nb` partials each: out[k] = sum_b partials[k*kRedGrid + b].
 __global__ void k_finalize(const double* __restrict__ partials, int nb, int nsum, double* __restrict__ out) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[kFinBlock / 64];
   for (int k = 0; k < nsum; ++k) {
     double v = 0.0;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) v += partials[k * kRedGrid + b];
@@ -39,7 +39,7 @@ __global__ void k_finalize(const double* __restrict__ partials, int nb, int nsum
   }
 }
 __global__ void k_finalize_max(const double* __restrict__ partials, int nb, int nsum, double* __restrict__ out) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[kFinBlock / 64];
   for (int k = 0; k < nsum; ++k) {
     double v = 0.0;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) v = nanmax(v, partials[k * kRedGrid + b]);
@@ -66,7 +66,7 @@ template <typename T>
 __global__ void k_rot(int64_t nedge, const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
                       const T* __restrict__ F, T* __restrict__ w, const uint8_t* __restrict__ mask,
                       const T* __restrict__ p, double* __restrict__ partials) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[kFinBlock / 64];
   double dot = 0.0;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nedge; e += (int64_t)gridDim.x * blockDim.x) {
     T acc = T(0);
@@ -259,7 +259,7 @@ template <typename T>
 __global__ void k_cg_begin(int64_t n, const T* __restrict__ res_e, const T* __restrict__ s,
                            const uint8_t* __restrict__ mask, const T* __restrict__ prec, T* __restrict__ r,
                            T* __restrict__ x, T* __restrict__ p, double* __restrict__ partials) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[kFinBlock / 64];
   double rr = 0, pr = 0, xx = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const T ri = mask[i] ? T(0) : res_e[i];
@@ -275,7 +275,7 @@ __global__ void k_cg_begin(int64_t n, const T* __restrict__ res_e, const T* __re
 }
 __global__ void k_cg_begin_fin(const double* __restrict__ partials, int nb, double tol1, double tol2,
                                double* __restrict__ scal) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[kFinBlock / 64];
   double v[3];
   for (int k = 0; k < 3; ++k) {
     double acc = 0.0;
@@ -290,7 +290,7 @@ __global__ void k_cg_begin_fin(const double* __restrict__ partials, int nb, doub
 }
 // alpha = eta / den (den != 0 else inf)     (ib_driver_np.py:389-392)
 __global__ void k_cg_alpha(const double* __restrict__ partials, int nb, double* __restrict__ scal) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[kFinBlock / 64];
   double acc = 0.0;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partials[b];
   double den = block_sum(acc, sh);
@@ -304,7 +304,7 @@ template <typename T>
 __global__ void k_cg_update(int64_t n, const double* __restrict__ scal, T* __restrict__ x, T* __restrict__ r,
                             const T* __restrict__ p, const T* __restrict__ w, const T* __restrict__ prec,
                             double* __restrict__ partials) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[kFinBlock / 64];
   const T alf = (T)scal[S_ALPHA];
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -319,7 +319,7 @@ __global__ void k_cg_update(int64_t n, const double* __restrict__ scal, T* __res
 }
 // eta_new and beta = eta_new / eta (eta != 0 else inf)   (ib_driver_np.py:396, 401)
 __global__ void k_cg_beta(const double* __restrict__ partials, int nb, double* __restrict__ scal) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[kFinBlock / 64];
   double acc = 0.0;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partials[b];
   double en = block_sum(acc, sh);
@@ -356,7 +356,7 @@ template <typename T>
 __global__ void k_true_res(int64_t n, const T* __restrict__ res_e, const T* __restrict__ img,
                            const uint8_t* __restrict__ mask, const T* __restrict__ prec, T* __restrict__ tr,
                            double* __restrict__ partials) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[kFinBlock / 64];
   double a = 0, b = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const T t = mask[i] ? T(0) : res_e[i] - img[i];
@@ -372,7 +372,7 @@ __global__ void k_true_res(int64_t n, const T* __restrict__ res_e, const T* __re
 template <typename T>
 __global__ void k_dot(int64_t n, const T* __restrict__ a, const T* __restrict__ b, const T* __restrict__ c,
                       double* __restrict__ partials) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[kFinBlock / 64];
   double acc = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double t = (double)a[i] * (double)b[i];
@@ -497,7 +497,7 @@ __global__ void k_interior(int64_t n, const int32_t* __restrict__ idx, const dou
                            const double* __restrict__ vol, const T* __restrict__ vc, double cx, double cy,
                            double cz, double radius, double width, int fixed, const double* __restrict__ fwv,
                            double* __restrict__ partials) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[kFinBlock / 64];
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const double px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
@@ -526,7 +526,7 @@ __global__ void k_interior(int64_t n, const int32_t* __restrict__ idx, const dou
 template <typename T>
 __global__ void k_fluid_load(int64_t n, const int32_t* __restrict__ idx, const double* __restrict__ pos,
                              const double* __restrict__ vol, const T* __restrict__ F, double* __restrict__ partials) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[kFinBlock / 64];
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int c = idx[i];
@@ -543,7 +543,7 @@ __global__ void k_fluid_load(int64_t n, const int32_t* __restrict__ idx, const d
 template <typename T>
 __global__ void k_diag_max(int64_t ncell, const T* __restrict__ div, const T* __restrict__ vc,
                            double* __restrict__ partials) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[kFinBlock / 64];
   double a = 0, b = 0;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x) {
     a = nanmax(a, fabs((double)div[c]));
@@ -578,7 +578,7 @@ int flow_mat_t(const pyib_mesh* m, const pyib_work* ws, double rou, double visc,
     const int nb = red_grid_for(m->nedge);
     k_rot<T><<<nb, kBlock, 0, st>>>(m->nedge, m->rot_ptr, m->rot_idx, face, w, mask_dir ? m->dir_mask : nullptr, p,
                                     ws->partials);
-    k_finalize<<<1, kBlock, 0, st>>>(ws->partials, nb, 1, dot_out);
+    k_finalize<<<1, kFinBlock, 0, st>>>(ws->partials, nb, 1, dot_out);
   } else {
     k_rot<T><<<grid_for(m->nedge), kBlock, 0, st>>>(m->nedge, m->rot_ptr, m->rot_idx, face, w,
                                                     mask_dir ? m->dir_mask : nullptr, nullptr, nullptr);
@@ -607,9 +607,9 @@ int cg_iterate_t(const pyib_mesh* m, const pyib_work* ws, const pyib_cg* cg, dou
                                                      (const T*)m->aol, (const T*)m->vol_ci, (T)visc, (T)rou, (T)hdt, ca, cb);
     k_integr<T><<<grid_for(m->nface), kBlock, 0, st>>>(m->nface, m->f2c, (const T*)m->rf1, (const T*)m->rf2, cb, face);
     k_rot<T><<<nb, kBlock, 0, st>>>(n, m->rot_ptr, m->rot_idx, face, w, m->dir_mask, p, ws->partials);
-    k_cg_alpha<<<1, kBlock, 0, st>>>(ws->partials, nb, cg->scal);
+    k_cg_alpha<<<1, kFinBlock, 0, st>>>(ws->partials, nb, cg->scal);
     k_cg_update<T><<<nb, kBlock, 0, st>>>(n, cg->scal, x, r, p, w, prec, ws->partials);
-    k_cg_beta<<<1, kBlock, 0, st>>>(ws->partials, nb, cg->scal);
+    k_cg_beta<<<1, kFinBlock, 0, st>>>(ws->partials, nb, cg->scal);
     if (!(skip_last && it == n_iter - 1)) {
       k_cg_dir<T><<<grid_for(n), kBlock, 0, st>>>(n, cg->scal, r, prec, p);
       k_cg_shift<<<1, 1, 0, st>>>(cg->scal);
@@ -744,7 +744,7 @@ int pyib_cg_begin(const pyib_mesh* m, const pyib_work* ws, const pyib_cg* cg, co
   else
     k_cg_begin<float><<<nb, kBlock, 0, st>>>(m->nedge, (const float*)res_e, (const float*)s, m->dir_mask,
         (const float*)cg->prec, (float*)cg->r, (float*)cg->x, (float*)cg->p, ws->partials);
-  k_cg_begin_fin<<<1, kBlock, 0, st>>>(ws->partials, nb, tol1, tol2, cg->scal);
+  k_cg_begin_fin<<<1, kFinBlock, 0, st>>>(ws->partials, nb, tol1, tol2, cg->scal);
   return check_launch("pyib_cg_begin");
 }
 
@@ -790,7 +790,7 @@ int pyib_cg_true_residual(const pyib_mesh* m, const pyib_work* ws, const pyib_cg
     k_true_res<float><<<nb, kBlock, 0, st>>>(m->nedge, (const float*)res_e, (const float*)tr, m->dir_mask,
                                              (const float*)cg->prec, (float*)tr, ws->partials);
   }
-  k_finalize<<<1, kBlock, 0, st>>>(ws->partials, nb, 2, out2);
+  k_finalize<<<1, kFinBlock, 0, st>>>(ws->partials, nb, 2, out2);
   return check_launch("pyib_cg_true_residual");
 }
 
@@ -830,7 +830,7 @@ int pyib_dot(int dtype, const void* a, const void* b, const void* c, int64_t n, 
     k_dot<double><<<nb, kBlock, 0, st>>>(n, (const double*)a, (const double*)b, (const double*)c, partials);
   else
     k_dot<float><<<nb, kBlock, 0, st>>>(n, (const float*)a, (const float*)b, (const float*)c, partials);
-  k_finalize<<<1, kBlock, 0, st>>>(partials, nb, 1, out);
+  k_finalize<<<1, kFinBlock, 0, st>>>(partials, nb, 1, out);
   return check_launch("pyib_dot");
 }
 
@@ -913,7 +913,7 @@ int pyib_interior_momenta(int dtype, int64_t n, const int32_t* idx, const double
   else
     k_interior<float><<<nb, kBlock, 0, st>>>(n, idx, pos, vol, (const float*)vc, cx, cy, cz, radius, width, fixed,
                                              fixed_wv, partials);
-  k_finalize<<<1, kBlock, 0, st>>>(partials, nb, 7, out7);
+  k_finalize<<<1, kFinBlock, 0, st>>>(partials, nb, 7, out7);
   return check_launch("pyib_interior_momenta");
 }
 
@@ -925,7 +925,7 @@ int pyib_fluid_load(int dtype, int64_t n, const int32_t* idx, const double* pos,
     k_fluid_load<double><<<nb, kBlock, 0, st>>>(n, idx, pos, vol, (const double*)F, partials);
   else
     k_fluid_load<float><<<nb, kBlock, 0, st>>>(n, idx, pos, vol, (const float*)F, partials);
-  k_finalize<<<1, kBlock, 0, st>>>(partials, nb, 6, out6);
+  k_finalize<<<1, kFinBlock, 0, st>>>(partials, nb, 6, out6);
   return check_launch("pyib_fluid_load");
 }
 
@@ -938,7 +938,7 @@ int pyib_diag_max(const pyib_mesh* m, const void* U, const void* vc, void* tmp_c
   double* partials = (double*)tmp_cell + m->ncell;
   const int nb = red_grid_for(m->ncell);
   k_diag_max<double><<<nb, kBlock, 0, st>>>(m->ncell, (const double*)tmp_cell, (const double*)vc, partials);
-  k_finalize_max<<<1, kBlock, 0, st>>>(partials, nb, 2, out2);
+  k_finalize_max<<<1, kFinBlock, 0, st>>>(partials, nb, 2, out2);
   return check_launch("pyib_diag_max");
 }
 

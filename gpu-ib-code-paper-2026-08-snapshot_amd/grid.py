@@ -16,11 +16,11 @@ import torch
 
 from . import _native as N
 from .core import dev_stream, ptr
-from .layout import MAX_LEVELS, build_layout, dir_mask_internal
+from .layout import MAX_LEVELS, build_layout, dir_mask_internal, to_sell
 
 
 class HipGrid:
-    def __init__(self, core, precisions=("float64",), margin=2, layout=None):
+    def __init__(self, core, precisions=("float64",), margin=1, layout=None):
         self.core = core
         dev = self.device = core.device
         self.lay = lay = layout if layout is not None else build_layout(core.mesh, core.geo, core.tables,
@@ -37,9 +37,10 @@ class HipGrid:
         self._to_ref_sig = t(lay.edge_sigma, torch.int8)
         self._comp = {}
         for name in ("g1", "vis", "g3"):
-            c = getattr(lay, name)
-            self._comp[name] = {"rows": t(c.rows, torch.int32), "ptr": t(c.ptr, torch.int32),
-                                "col": t(c.col, torch.int32), "n": int(c.rows.shape[0]), "coef64": c.coef}
+            c = to_sell(getattr(lay, name))
+            self._comp[name] = {"rows": t(c.rows, torch.int32), "width": t(c.width, torch.int32),
+                                "sptr": t(c.sptr, torch.int64), "col": t(c.col, torch.int32),
+                                "n": int(c.width.shape[0]), "coef64": c.coef}
         self._coef = {}
         self._struct = {}
         self.work = {}
@@ -76,12 +77,14 @@ class HipGrid:
             L.h = lv.h
         g.cell_reg, g.edge_reg, g.dir = ptr(self._cell_reg), ptr(self._edge_reg), ptr(self._dir)
         c, k = self._comp, self._coef[prec]
-        g.g1_nrows, g.g1_row, g.g1_ptr, g.g1_col, g.g1_coef = (c["g1"]["n"], ptr(c["g1"]["rows"]), ptr(c["g1"]["ptr"]),
-                                                                ptr(c["g1"]["col"]), ptr(k["g1"]))
-        g.v_nrows, g.v_row, g.v_ptr, g.v_col, g.v_coef = (c["vis"]["n"], ptr(c["vis"]["rows"]), ptr(c["vis"]["ptr"]),
-                                                           ptr(c["vis"]["col"]), ptr(k["vis"]))
-        g.g3_nrows, g.g3_row, g.g3_ptr, g.g3_col, g.g3_coef = (c["g3"]["n"], ptr(c["g3"]["rows"]), ptr(c["g3"]["ptr"]),
-                                                                ptr(c["g3"]["col"]), ptr(k["g3"]))
+        for pre, name in (("g1", "g1"), ("v", "vis"), ("g3", "g3")):
+            cc = c[name]
+            setattr(g, pre + "_nslice", cc["n"])
+            setattr(g, pre + "_row", ptr(cc["rows"]))
+            setattr(g, pre + "_width", ptr(cc["width"]))
+            setattr(g, pre + "_sptr", ptr(cc["sptr"]))
+            setattr(g, pre + "_col", ptr(cc["col"]))
+            setattr(g, pre + "_coef", ptr(k[name]))
         return g
 
     def set_dir_mask(self, dir_mask_ref):

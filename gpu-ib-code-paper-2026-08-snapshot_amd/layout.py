@@ -85,11 +85,15 @@ def _cells_of_level(lv):
     return m
 
 
-def build_layout(mesh, geo, tables, margin=2, tol=1e-13):
+def build_layout(mesh, geo, tables, margin=1, tol=1e-13, demote_neighbours=False):
     """Internal layout + classification.  ``tol`` is the relative tolerance of the
     closed-form coefficient check (0: bitwise).  The default keeps rows whose
     geometry differs from the closed form only by the rounding of non-dyadic
-    node coordinates (~1e-16 relative)."""
+    node coordinates (~1e-16 relative).  ``margin`` cells next to the box and the
+    hole are never regular (the geometry check demotes the first layer anyway; one
+    cell keeps every stencil read inside its level block).  ``demote_neighbours``
+    also demotes the neighbours of geometrically irregular cells (not needed for
+    exactness: the V stencil reads its neighbours' values, whichever pass wrote them)."""
     N = mesh.n_layers
     if N > MAX_LEVELS:
         raise ValueError(f"at most {MAX_LEVELS} levels")
@@ -159,7 +163,7 @@ def build_layout(mesh, geo, tables, margin=2, tol=1e-13):
     lay = StructuredLayout(levels, ncell_i, nedge_i, cell_map, edge_map, sigma, edge_inv, sigma_int,
                            np.zeros(ncell_i, np.uint8), np.zeros(nedge_i, np.uint8))
     _classify(lay, margin)
-    _verify_regular(lay, mesh, geo, tables, tol)
+    _verify_regular(lay, mesh, geo, tables, tol, demote_neighbours)
     _composites(lay, geo, tables)
     return lay
 
@@ -207,7 +211,7 @@ def _close(a, b, tol, scale):
     return np.abs(a - b) <= tol * scale if tol > 0 else a == b
 
 
-def _verify_regular(lay, mesh, geo, tables, tol):
+def _verify_regular(lay, mesh, geo, tables, tol, demote_neighbours=True):
     """Demote regular rows whose surrounding geometry differs from the closed form."""
     ncell = geo.ncell
     cinv = np.full(lay.ncell, -1, np.int64)
@@ -245,7 +249,8 @@ def _verify_regular(lay, mesh, geo, tables, tol):
     inner = oth < ncell
     nb_bad_idx = own[inner][bad_cell[oth[inner]]]
     nb_bad[nb_bad_idx] = True
-    demote[lay.cell_map[nb_bad]] = True
+    if demote_neighbours:
+        demote[lay.cell_map[nb_bad]] = True
     before = int(lay.reg_cell.sum())
     lay.reg_cell[demote] = 0
     # edges: re-derive from the (possibly demoted) cells
@@ -347,6 +352,64 @@ def _composites(lay, geo, t):
     lay.g3 = Composite(irr_edges_int.astype(np.int32), ptr, col, cf)
     lay.stats.update(irregular_cells=int(irr_cells_int.shape[0]), irregular_edges=int(irr_edges_int.shape[0]),
                      g1_nnz=int(lay.g1.col.shape[0]), v_nnz=int(lay.vis.col.shape[0]), g3_nnz=int(lay.g3.col.shape[0]))
+
+
+SLICE = 64   # rows per slice of the sliced-ELL composites: one wavefront
+
+
+@dataclass
+class SellComposite:
+    """Composite rows in sliced ELL (slices of SLICE rows, column-major inside a slice)."""
+    rows: np.ndarray            # (nslice*SLICE,) int32 internal row id, -1 on padding lanes
+    width: np.ndarray           # (nslice,) int32 entries per lane
+    sptr: np.ndarray            # (nslice,) int64 first entry of the slice
+    col: np.ndarray             # int32, 0 on padding
+    coef: np.ndarray            # float64 (entries,) or (entries, 3), 0 on padding
+
+
+def to_sell(comp, C=SLICE):
+    """CSR composite -> sliced ELL.  Rows are ordered by decreasing length (stable, so rows
+    of one length keep their spatial order) to keep the padding small; each lane keeps its
+    row's entries in CSR order, so the per-row sums are unchanged."""
+    n = int(comp.rows.shape[0])
+    width = np.diff(comp.ptr).astype(np.int64)
+    order = np.argsort(-width, kind="stable")
+    nslice = (n + C - 1) // C
+    w_sorted = np.zeros(nslice * C, np.int64)
+    w_sorted[:n] = width[order]
+    sw = w_sorted.reshape(nslice, C).max(axis=1) if nslice else np.zeros(0, np.int64)
+    sptr = np.zeros(nslice, np.int64)
+    if nslice:
+        np.cumsum(sw[:-1] * C, out=sptr[1:])
+    total = int((sw * C).sum())
+    pos = np.empty(n, np.int64)
+    pos[order] = np.arange(n)
+    ent_row = np.repeat(np.arange(n), width)
+    ent_j = np.arange(int(width.sum())) - np.repeat(comp.ptr[:-1].astype(np.int64), width)
+    p = pos[ent_row]
+    dst = sptr[p // C] + ent_j * C + p % C
+    col = np.zeros(total, np.int32)
+    col[dst] = comp.col
+    coef = np.zeros((total,) + comp.coef.shape[1:], comp.coef.dtype)
+    coef[dst] = comp.coef
+    rows = np.full(nslice * C, -1, np.int32)
+    rows[:n] = comp.rows[order]
+    return SellComposite(rows, sw.astype(np.int32), sptr, col, coef)
+
+
+def sell_apply(sc, x_gather, C=SLICE):
+    """Host emulation of a sliced-ELL row sum: out[row] = sum_j coef * x_gather(col), summed
+    sequentially in entry order as the kernels do (padding entries add +0)."""
+    out = {}
+    for s in range(sc.width.shape[0]):
+        for l in range(C):
+            r = int(sc.rows[s * C + l])
+            if r < 0:
+                continue
+            idx = sc.sptr[s] + np.arange(int(sc.width[s])) * C + l
+            terms = sc.coef[idx] * x_gather(sc.col[idx])
+            out[r] = np.cumsum(terms, axis=0)[-1] if terms.shape[0] else np.zeros(terms.shape[1:])
+    return out
 
 
 def dir_mask_internal(lay, dir_mask_ref):

@@ -75,7 +75,7 @@ typedef struct pyib_work {
   double *partials;         /* >= PYIB_MAX_PARTIALS doubles */
 } pyib_work;
 
-#define PYIB_MAX_PARTIALS 2048
+#define PYIB_MAX_PARTIALS 65536
 
 /* CG state; vectors have the mesh dtype.  scal is a device array of
  * PYIB_NSCAL doubles: [eta, eta_new, den, alpha, beta, rr, err_tot, l2_init]. */
@@ -105,10 +105,15 @@ typedef struct pyib_grid {
   const uint8_t *cell_reg;  /* ncell: row computed by the stencil */
   const uint8_t *edge_reg;  /* nedge: row computed by the stencil */
   const uint8_t *dir;       /* nedge: Dirichlet edge or empty slot (residual forced to 0) */
-  /* composite rows of the other (interface/boundary) rows, CSR */
-  int64_t g1_nrows; const int32_t *g1_row, *g1_ptr, *g1_col; const void *g1_coef; /* coef nnz x 3 */
-  int64_t v_nrows;  const int32_t *v_row, *v_ptr, *v_col;  const void *v_coef;    /* coef nnz */
-  int64_t g3_nrows; const int32_t *g3_row, *g3_ptr, *g3_col; const void *g3_coef; /* coef nnz x 3 */
+  /* composite rows of the other (interface/boundary) rows in sliced ELL: slices of 64
+   * rows (one wavefront), entry j of lane l of slice s at sptr[s] + 64*j + l, width[s]
+   * entries per lane, row[64*s + l] = internal row id or -1 (padding, col 0, coef 0) */
+  int64_t g1_nslice; const int32_t *g1_row, *g1_width; const int64_t *g1_sptr;
+  const int32_t *g1_col; const void *g1_coef;   /* coef entries x 3 */
+  int64_t v_nslice;  const int32_t *v_row, *v_width;  const int64_t *v_sptr;
+  const int32_t *v_col;  const void *v_coef;    /* coef entries */
+  int64_t g3_nslice; const int32_t *g3_row, *g3_width; const int64_t *g3_sptr;
+  const int32_t *g3_col; const void *g3_coef;   /* coef entries x 3 */
 } pyib_grid;
 
 const char *pyib_last_error(void);
@@ -184,6 +189,9 @@ int pyib_permute(int dtype_in, const void *in, int dtype_out, void *out, int64_t
  * with the inlet edges zeroed. */
 int pyib_calc_vel_adjoint(const pyib_mesh *m, const int32_t *face_ghost, double *g_vc,
                           const double *g_U_in, double *g_U, double *g_s, void *stream);
+/* pyib_curl_rhs^T: g_res_c (ncell,3) = integr_c2f^T curl g_edge  (uses ws->face) */
+int pyib_curl_rhs_adjoint(const pyib_mesh *m, const pyib_work *ws, const double *g_edge, double *g_res_c,
+                          void *stream);
 /* momentum residual^T: g_vc (ncell+nghost, 3) += d out/d vc^T g_out, g_U += d out/d U^T g_out */
 int pyib_momentum_residual_adjoint(const pyib_mesh *m, const int32_t *face_ghost, const double *U,
                                    const double *vc, double visc, int has_base, double a_vc,

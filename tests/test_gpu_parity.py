@@ -167,6 +167,7 @@ def test_reference_golden_trajectory(name, steps):
     rt, s, vc, rows = _run_hip(cfg, steps)
     ort = OC.build_runtime(cfg, sparse=False)
     os_mf = OC.run(ort, steps)[0]
+    fl_mf = np.asarray(ort.driver.ib_force_L)
     with golden(f"traj_{name}.npz") as g:
         cols = [str(c) for c in g["columns"]]
         hist = g["history"]
@@ -175,7 +176,10 @@ def test_reference_golden_trajectory(name, steps):
         floor = rel(os_mf, g["s"])
         print(f"[gpu] {name}: s vs reference {rel(H(s), g['s']):.2e} (matrix-free oracle vs reference {floor:.2e})")
         assert rel(H(s), g["s"]) <= parity_tol(floor)
-        assert rel(H(rt.driver.ib_force_L), g["ib_force_L"]) < 1e-10
+        ffl = rel(fl_mf, g["ib_force_L"])
+        print(f"[gpu] {name}: Lagrangian force vs reference {rel(H(rt.driver.ib_force_L), g['ib_force_L']):.2e} "
+              f"(matrix-free oracle vs reference {ffl:.2e})")
+        assert rel(H(rt.driver.ib_force_L), g["ib_force_L"]) <= parity_tol(ffl, base=1e-10)
         fr = force_rel(force_history(rows), force_history((hist, cols)))
         fc = force_rel(force_history(rows, ("force_fluid_x", "Cd")), force_history((hist, cols), ("force_fluid_x", "Cd")))
         print(f"[gpu] {name}: force history rel {fr:.2e}, (force_fluid_x, Cd) rel {fc:.2e}")

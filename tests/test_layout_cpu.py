@@ -113,3 +113,22 @@ def test_round_trip_numbering():
     # level-0 cells keep PyIB's numbering (index_cell is the identity there)
     n0 = m.layer_cell_counts[0]
     assert np.array_equal(lay.cell_map[:n0], np.arange(n0))
+
+
+def test_sliced_ell_keeps_every_row_sum():
+    """The sliced-ELL copy of each composite gives the CSR row sums bit for bit."""
+    from pyib_hip.layout import sell_apply, to_sell
+    m = PM.make_nested_mesh(0.0625, PM.nested_box_boxes([1.5, 1.25, 1.25], 2))
+    g = Geometry(m)
+    lay = build_layout(m, g, build_tables(g))
+    x = np.random.default_rng(3).standard_normal(max(lay.ncell, lay.nedge))
+    for comp in (lay.g1, lay.vis, lay.g3):
+        sc = to_sell(comp)
+        assert sc.rows.shape[0] % 64 == 0 and np.all(sc.col[sc.coef.reshape(sc.col.shape[0], -1).any(axis=1) == 0]
+                                                     >= 0)
+        got = sell_apply(sc, lambda c: x[c][:, None] if comp.coef.ndim == 2 else x[c])
+        assert sorted(got) == sorted(int(r) for r in comp.rows)
+        for k, r in enumerate(comp.rows):
+            a, b = comp.ptr[k], comp.ptr[k + 1]
+            terms = comp.coef[a:b] * (x[comp.col[a:b]][:, None] if comp.coef.ndim == 2 else x[comp.col[a:b]])
+            assert np.array_equal(got[int(r)], np.cumsum(terms, axis=0)[-1])      # sequential, as the kernels
