@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_EDGE_ITER = {"float64": 104.0, "float32": 52.0}   # 13 words per edge per PCG iteration
 
 
-def headline_config(steps=10, precision="float64"):
+def headline_config(steps=10, precision="float64", mixed_inner_tol=None):
     f = 3.6 / (2.0 * math.pi)
     T = 1.0 / f
     return {
@@ -50,16 +50,22 @@ def headline_config(steps=10, precision="float64"):
         "solver": {"sparse": False, "cg_check_interval": 20, "cg_max_iterations": 200000,
                    "fail_on_cg_nonconvergence": True},
         "output": {"sample_every": 1, "checkpoint_every": 0, "save_field": False},
-    }
+    } | ({} if mixed_inner_tol is None else {"solver": {"sparse": False, "cg_check_interval": 20,
+                                                      "cg_max_iterations": 200000,
+                                                      "fail_on_cg_nonconvergence": True,
+                                                      "mixed_inner_tolerance": float(mixed_inner_tol)}})
 
 
 class CGTimer:
     """HIP events (torch's current stream, where every kernel is launched) around each PCG block."""
 
-    def __init__(self, torch):
+    def __init__(self, torch, log=None, every_s=20.0):
         self.torch = torch
         self.pairs = []
         self.iters = 0
+        self.log = log
+        self.every_s = every_s
+        self._last = time.perf_counter()
 
     def begin(self):
         e = self.torch.cuda.Event(enable_timing=True)
@@ -70,6 +76,11 @@ class CGTimer:
         e = self.torch.cuda.Event(enable_timing=True)
         e.record()
         self.pairs.append((start, e, n))
+        self.iters += int(n[0])
+        now = time.perf_counter()
+        if self.log is not None and now - self._last > self.every_s:
+            self._last = now
+            self.log(f"[bench]   ... {self.iters} PCG iterations so far ({n[1]})")
 
     def reset(self):
         self.pairs, self.iters = [], 0
@@ -93,17 +104,17 @@ def build_mesh(log):
     return mesh
 
 
-def run_gpu(precision, steps, warmup, log, mesh):
+def run_gpu(precision, steps, warmup, log, mesh, mixed_inner_tol=None):
     import torch
     from pyib_hip import case as HC
 
-    cfg = HC.validate_config(headline_config(steps + warmup, precision))
+    cfg = HC.validate_config(headline_config(steps + warmup, precision, mixed_inner_tol))
     t1 = time.perf_counter()
     rt = HC.build_runtime(cfg, mesh=mesh)
     log(f"[bench] runtime ({precision}) {rt.core.nedge:,} edges, {rt.reference_markers.shape[0]} markers "
         f"in {time.perf_counter() - t1:.1f} s")
     drv = rt.driver
-    timer = CGTimer(torch)
+    timer = CGTimer(torch, log)
     drv.cg_timer = timer
     s, s_old, vc = HC.initial_state(rt)
     iters, forces = [], []
@@ -235,6 +246,8 @@ def main(argv=None):
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=20.0)
     p.add_argument("--kernels-only", type=int, default=0, help="profiling: N PCG iterations + a copy kernel")
+    p.add_argument("--mixed-inner-tol", type=float, default=None,
+                   help="float32 residual reduction between float64 reliable updates (default: the schema's)")
     a = p.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus})",
@@ -247,7 +260,7 @@ def main(argv=None):
     results = {}
     mesh = build_mesh(log)
     for prec in precs:
-        results[prec] = run_gpu(prec, a.steps, a.warmup, log, mesh)
+        results[prec] = run_gpu(prec, a.steps, a.warmup, log, mesh, a.mixed_inner_tol)
     head = "mixed" if "mixed" in results else "float64"
     r = results[head]
     inner_prec = "float32" if head == "mixed" else "float64"

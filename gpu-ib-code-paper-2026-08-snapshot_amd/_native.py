@@ -11,7 +11,7 @@ import os
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpyib_hip.so")
 
 F64, F32 = 0, 1
-MAX_PARTIALS = 65536
+MAX_PARTIALS = 8192
 NSCAL = 8
 S_ETA, S_ETA_NEW, S_DEN, S_ALPHA, S_BETA, S_RR, S_ERR, S_L2I = range(8)
 
@@ -90,6 +90,7 @@ _SIGS = {
     "pyib_permute": (C.c_int, [C.c_int, vp, C.c_int, vp, C.c_int64, vp, vp, vp]),
     "pyib_extrapolate": (C.c_int, [C.c_int64, vp, vp, C.c_double, C.c_double, vp, vp]),
     "pyib_convert": (C.c_int, [C.c_int, vp, C.c_int, vp, C.c_int64, vp]),
+    "pyib_cg_replace_residual": (C.c_int, [vp, vp, vp]),
     "pyib_add_into_f64": (C.c_int, [C.c_int, vp, vp, C.c_int64, vp]),
     "pyib_dot": (C.c_int, [C.c_int, vp, vp, vp, C.c_int64, vp, vp, vp]),
     "pyib_axpby": (C.c_int, [C.c_int, C.c_double, vp, C.c_double, vp, vp, C.c_int64, vp]),
@@ -123,11 +124,13 @@ EXPORTED = tuple(_SIGS)
 _lib = None
 
 
-def load(path: str = LIB_PATH):
-    """Load the HIP library (raises if it has not been built)."""
+def load(path: str = None):
+    """Load the HIP library (raises if it has not been built).  PYIB_HIP_LIBRARY overrides
+    the in-tree path (a library built elsewhere from these sources)."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or os.environ.get("PYIB_HIP_LIBRARY") or LIB_PATH
     if not os.path.exists(path):
         raise RuntimeError(f"HIP library {path} is missing; run __graft_entry__.build() (no CPU fallback exists)")
     lib = C.CDLL(path)

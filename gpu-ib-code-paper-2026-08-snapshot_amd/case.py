@@ -9,7 +9,8 @@ validated resume (914-1121) and signal-safe stop.  The schema grows only
 additively: ``runtime.device`` accepts ``"hip"``, ``runtime.precision``
 accepts ``"mixed"``, ``mesh.kind`` accepts ``"nested_box"`` (finest box
 half extents in diameters, each coarser level doubling), and
-``solver.mixed_inner_tolerance`` sets the float32 inner reduction.
+``solver.mixed_inner_tolerance`` sets the float32 residual reduction between
+float64 reliable updates (default 0.1).
 Errors keep the reference's types: ``CaseConfigError`` for configuration,
 checkpoint and history problems, ``RuntimeError`` for solver and invariant
 failures, ``FloatingPointError`` for non-finite diagnostics and
@@ -478,7 +479,7 @@ def _setup_objects(config, mesh):
         params["fingerprint"] = hashlib.sha256(np.ascontiguousarray(mesh.nodes).tobytes()).hexdigest()
     geo = setup_cache.cached("geometry", params, lambda: Geometry(mesh), Geometry)
     tables = setup_cache.cached("tables", params, lambda: build_tables(geo), Tables)
-    lay = setup_cache.cached("layout", dict(params, layout_rules="margin1-geometric"),
+    lay = setup_cache.cached("layout", dict(params, layout_rules="per-pass-regularity"),
                              lambda: build_layout(mesh, geo, tables), StructuredLayout,
                              {"Composite": Composite, "Level": Level})
     return geo, tables, lay
@@ -549,7 +550,7 @@ def build_runtime(config, mesh=None, device="cuda"):
     drv = HipDriver(core, mesh, visc=rho * nu, rou=rho, cg_max_iterations=int(sv["cg_max_iterations"]),
                     fail_on_cg_nonconvergence=bool(sv["fail_on_cg_nonconvergence"]),
                     cg_check_interval=int(sv["cg_check_interval"]), precision=precision,
-                    mixed_inner_tol=float(sv.get("mixed_inner_tolerance", 1e-4)), layout=lay)
+                    mixed_inner_tol=float(sv.get("mixed_inner_tolerance", 0.1)), layout=lay)
     if uses_cavity(config):
         drv.set_cavity(U_lid=0.0, spanwise_zerograd=False)
     dt = float(config["time"]["dt"])

@@ -34,15 +34,6 @@ inline int red_grid_for(int64_t n) {
   return static_cast<int>(g > kRedGrid ? kRedGrid : g);
 }
 
-// Block index remapped so that each of the 8 XCDs (which receive workgroups
-// round-robin by blockIdx) works on one contiguous 1/8 of the index range: the
-// stencil neighbours of a block (+-1 row, +-1 plane) then sit in the same XCD's L2.
-__device__ __forceinline__ int64_t xcd_block() {
-  const int64_t nb = gridDim.x, b = blockIdx.x;
-  const int64_t q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
-  return x * q + (x < r ? x : r) + i;
-}
-
 template <typename T>
 __device__ __forceinline__ T sload(const T* __restrict__ x, int v) {
   return v >= 0 ? x[v] : -x[-v - 1];

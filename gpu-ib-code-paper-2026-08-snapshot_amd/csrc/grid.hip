@@ -3,11 +3,11 @@
 // The CG of Driver.helm_solve (ib_driver_np.py:311-410) runs on the internal
 // layout: per level, dense x/y/z edge blocks and a cell block, +axis oriented.
 // A p = rot(integr(V(interp(curl p)))) (ib_core_np.py:213-220) is applied in
-// three passes; each pass computes the rows flagged regular with the
-// constant-coefficient stencil of its level (one thread per row, the fastest
-// index k on consecutive lanes so every stencil read is a coalesced stream)
-// and the remaining rows from explicit composite rows (level interfaces,
-// outer boundary).  Cell vectors are stored component-major (SoA).
+// three passes.  Each pass first runs the constant-coefficient stencil of each
+// level over its whole box (2.5D tiles staged in LDS, the fastest index k on
+// consecutive lanes), then the composite rows (level interfaces, outer boundary,
+// sliced ELL) overwrite the rows the stencil does not describe.  Cell vectors
+// are stored component-major (SoA).
 //
 //   G1  vc_d(c)  = (U_d(lo) + U_d(hi)) / (2h^2)       U = discrete curl of p
 //   V   y(c)     = rou x - (hdt/h^3)(a - b),  a = sum_up (h visc)(x_n - x), b = sum_dn (h visc)(x - x_n)
@@ -21,7 +21,6 @@ struct Lev {
   int nx, ny, nz;
   int64_t cell_off, e0, e1, e2;
   double h;
-  double inv_nz, inv_ny, inv_nz1, inv_ny1;   // reciprocals for the index decode
 };
 
 struct Levels {
@@ -34,66 +33,12 @@ Levels levels_of(const pyib_grid* g) {
   L.n = g->nlevels;
   for (int i = 0; i < g->nlevels; ++i) {
     const pyib_level& s = g->lev[i];
-    L.l[i] = Lev{s.nx, s.ny, s.nz, s.cell_off, s.e_off[0], s.e_off[1], s.e_off[2], s.h,
-                 1.0 / s.nz, 1.0 / s.ny, 1.0 / (s.nz + 1), 1.0 / (s.ny + 1)};
+    L.l[i] = Lev{s.nx, s.ny, s.nz, s.cell_off, s.e_off[0], s.e_off[1], s.e_off[2], s.h};
   }
   return L;
 }
 
-__device__ __forceinline__ int level_of_cell(const Levels& L, int64_t c) {
-  int k = 0;
-  while (k + 1 < L.n && c >= L.l[k + 1].cell_off) ++k;
-  return k;
-}
-__device__ __forceinline__ int level_of_edge(const Levels& L, int64_t e) {
-  int k = 0;
-  while (k + 1 < L.n && e >= L.l[k + 1].e0) ++k;
-  return k;
-}
-
-// q / d for 0 <= q < 2^31 through a double reciprocal, corrected to the exact quotient
-__device__ __forceinline__ int qdiv(int q, int d, double inv) {
-  int t = (int)((double)q * inv);
-  const int r = q - t * d;
-  t += (r >= d) ? 1 : 0;
-  t -= (r < 0) ? 1 : 0;
-  return t;
-}
-// (i, j, k) of offset q in a C-ordered (., ny, nz) block
-__device__ __forceinline__ void decode(int q, int ny, int nz, double inv_ny, double inv_nz, int& i, int& j, int& k) {
-  const int t = qdiv(q, nz, inv_nz);
-  k = q - t * nz;
-  i = qdiv(t, ny, inv_ny);
-  j = t - i * ny;
-}
-
 // ---------------------------------------------------------------- G1
-template <typename T>
-__global__ void k_sg1(int64_t ncell, Levels L, const uint8_t* __restrict__ reg, const T* __restrict__ p,
-                      T* __restrict__ vc) {
-  for (int64_t c = xcd_block() * blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x) {
-    if (!reg[c]) continue;
-    const Lev& v = L.l[level_of_cell(L, c)];
-    int i, j, k;
-    decode((int)(c - v.cell_off), v.ny, v.nz, v.inv_ny, v.inv_nz, i, j, k);
-    const int ny1 = v.ny + 1, nz1 = v.nz + 1;
-    // x-edges (nx, ny+1, nz+1), y-edges (nx+1, ny, nz+1), z-edges (nx+1, ny+1, nz)
-    auto PX = [&](int a, int b, int d) { return p[v.e0 + ((int64_t)a * ny1 + b) * nz1 + d]; };
-    auto PY = [&](int a, int b, int d) { return p[v.e1 + ((int64_t)a * v.ny + b) * nz1 + d]; };
-    auto PZ = [&](int a, int b, int d) { return p[v.e2 + ((int64_t)a * ny1 + b) * v.nz + d]; };
-    const T ux0 = (PY(i, j, k) - PY(i, j, k + 1)) + (PZ(i, j + 1, k) - PZ(i, j, k));
-    const T ux1 = (PY(i + 1, j, k) - PY(i + 1, j, k + 1)) + (PZ(i + 1, j + 1, k) - PZ(i + 1, j, k));
-    const T uy0 = (PX(i, j, k + 1) - PX(i, j, k)) - (PZ(i + 1, j, k) - PZ(i, j, k));
-    const T uy1 = (PX(i, j + 1, k + 1) - PX(i, j + 1, k)) - (PZ(i + 1, j + 1, k) - PZ(i, j + 1, k));
-    const T uz0 = (PY(i + 1, j, k) - PY(i, j, k)) - (PX(i, j + 1, k) - PX(i, j, k));
-    const T uz1 = (PY(i + 1, j, k + 1) - PY(i, j, k + 1)) - (PX(i, j + 1, k + 1) - PX(i, j, k + 1));
-    const T cf = (T)(1.0 / (2.0 * v.h * v.h));
-    vc[c] = (ux0 + ux1) * cf;
-    vc[ncell + c] = (uy0 + uy1) * cf;
-    vc[2 * ncell + c] = (uz0 + uz1) * cf;
-  }
-}
-
 // composite rows in sliced ELL, one wavefront per slice of 64 rows (lane = row):
 // vc_d(row) = sum_j coef_d * p[col]
 template <typename T>
@@ -119,28 +64,6 @@ __global__ void k_sell_g1(int64_t nslice, int64_t ncell, const int32_t* __restri
 }
 
 // ----------------------------------------------------------------- V
-template <typename T>
-__global__ void k_sv(int64_t ncell, Levels L, const uint8_t* __restrict__ reg, T rou, T visc, T hdt,
-                     const T* __restrict__ x, T* __restrict__ y) {
-  for (int64_t c = xcd_block() * blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x) {
-    if (!reg[c]) continue;
-    const Lev& v = L.l[level_of_cell(L, c)];
-    const int64_t si = (int64_t)v.ny * v.nz, sj = v.nz;
-    const T kA = (T)v.h * visc;
-    const T hv = hdt * (T)(1.0 / (v.h * v.h * v.h));
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const T* xd = x + d * ncell;
-      const T xc = xd[c];
-      T a = kA * (xd[c + si] - xc) + kA * (xd[c + sj] - xc);
-      a = a + kA * (xd[c + 1] - xc);
-      T b = kA * (xc - xd[c - si]) + kA * (xc - xd[c - sj]);
-      b = b + kA * (xc - xd[c - 1]);
-      y[d * ncell + c] = rou * xc - hv * (a - b);
-    }
-  }
-}
-
 template <typename T>
 __global__ void k_sell_v(int64_t nslice, int64_t ncell, const int32_t* __restrict__ row,
                          const int32_t* __restrict__ width, const int64_t* __restrict__ sptr,
@@ -169,78 +92,6 @@ __global__ void k_sell_v(int64_t nslice, int64_t ncell, const int32_t* __restric
 }
 
 // ---------------------------------------------------------------- G3
-// w_e of one regular edge e of level v
-template <typename T>
-__device__ __forceinline__ T sg3_value(const Lev& v, int64_t e, int64_t ncell, const T* __restrict__ y) {
-  const int64_t si = (int64_t)v.ny * v.nz, sj = v.nz;
-  const T hh = (T)(0.5 * v.h);
-  const T* yx = y;
-  const T* yy = y + ncell;
-  const T* yz = y + 2 * ncell;
-  int dir, i, j, k;
-  if (e < v.e1) {
-    dir = 0;
-    decode((int)(e - v.e0), v.ny + 1, v.nz + 1, v.inv_ny1, v.inv_nz1, i, j, k);
-  } else if (e < v.e2) {
-    dir = 1;
-    decode((int)(e - v.e1), v.ny, v.nz + 1, v.inv_ny, v.inv_nz1, i, j, k);
-  } else {
-    dir = 2;
-    decode((int)(e - v.e2), v.ny + 1, v.nz, v.inv_ny1, v.inv_nz, i, j, k);
-  }
-  const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;   // cell (i,j,k) of this level
-  T val;
-  if (dir == 0) {
-    // -F_y(i,j,k) + F_y(i,j,k-1) + F_z(i,j,k) - F_z(i,j-1,k)
-    const T fy0 = hh * (yy[c - sj] + yy[c]);
-    const T fy1 = hh * (yy[c - sj - 1] + yy[c - 1]);
-    const T fz0 = hh * (yz[c - 1] + yz[c]);
-    const T fz1 = hh * (yz[c - sj - 1] + yz[c - sj]);
-    val = (fy1 - fy0) + (fz0 - fz1);
-  } else if (dir == 1) {
-    // F_x(i,j,k) - F_x(i,j,k-1) - F_z(i,j,k) + F_z(i-1,j,k)
-    const T fx0 = hh * (yx[c - si] + yx[c]);
-    const T fx1 = hh * (yx[c - si - 1] + yx[c - 1]);
-    const T fz0 = hh * (yz[c - 1] + yz[c]);
-    const T fz1 = hh * (yz[c - si - 1] + yz[c - si]);
-    val = (fx0 - fx1) + (fz1 - fz0);
-  } else {
-    // -F_x(i,j,k) + F_x(i,j-1,k) + F_y(i,j,k) - F_y(i-1,j,k)
-    const T fx0 = hh * (yx[c - si] + yx[c]);
-    const T fx1 = hh * (yx[c - si - sj] + yx[c - sj]);
-    const T fy0 = hh * (yy[c - sj] + yy[c]);
-    const T fy1 = hh * (yy[c - si - sj] + yy[c - si]);
-    val = (fx1 - fx0) + (fy0 - fy1);
-  }
-  return val;
-}
-
-// block partials of p.w over the rows this kernel writes; each block takes chunks of
-// kBlock*kPerThread consecutive edges (XCD-remapped), kPerThread coalesced rows per thread
-template <typename T>
-__global__ void k_sg3(int64_t nedge, int64_t ncell, Levels L, const uint8_t* __restrict__ reg,
-                      const T* __restrict__ y, T* __restrict__ w, const T* __restrict__ p,
-                      double* __restrict__ partials) {
-  __shared__ double sh[kFinBlock / 64];
-  double dot = 0.0;
-  const int64_t chunk = (int64_t)blockDim.x * kPerThread;
-  for (int64_t base = xcd_block() * chunk; base < nedge; base += (int64_t)gridDim.x * chunk) {
-#pragma unroll
-    for (int u = 0; u < kPerThread; ++u) {
-      const int64_t e = base + (int64_t)u * blockDim.x + threadIdx.x;
-      if (e < nedge && reg[e]) {
-        const T val = sg3_value(L.l[level_of_edge(L, e)], e, ncell, y);
-        w[e] = val;
-        if (p) dot += (double)p[e] * (double)val;
-      }
-    }
-  }
-  if (partials) {
-    double r = block_sum(dot, sh);
-    if (threadIdx.x == 0) partials[blockIdx.x] = r;
-  }
-}
-
 template <typename T>
 __global__ void k_sell_g3(int64_t nslice, int64_t ncell, const int32_t* __restrict__ row,
                           const int32_t* __restrict__ width, const int64_t* __restrict__ sptr,
@@ -273,6 +124,235 @@ __global__ void k_sell_g3(int64_t nslice, int64_t ncell, const int32_t* __restri
   if (partials) {
     double rr = block_sum(dot, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = rr;
+  }
+}
+
+// ------------------------------------------------- 2.5D tiled stencil passes
+// Each block owns a TJ x TK tile of one level's (j, k) plane and marches over a chunk of
+// CI planes in i (grid: x = tiles of the largest level, y = i-chunks, z = level).  The
+// planes it needs (with their halo in j and k) are staged once in LDS per step, so every
+// operand is read from memory about once instead of once per stencil tap.  Values outside
+// the level box are staged as 0: only rows away from the box edge are regular, and the
+// composite pass that follows overwrites the others.
+constexpr int TJ = 8, TK = 32, CI = 16;        // 256 threads: tj = tid / TK, tk = tid % TK
+
+// grid of the tiled passes; *ci = planes per block (CI, doubled until the block count fits
+// the reduction partials)
+inline dim3 tile_grid(const Levels& L, int* ci) {
+  int my = 1, mz = 1, mx = 1;
+  for (int i = 0; i < L.n; ++i) {
+    my = L.l[i].ny > my ? L.l[i].ny : my;
+    mz = L.l[i].nz > mz ? L.l[i].nz : mz;
+    mx = L.l[i].nx > mx ? L.l[i].nx : mx;
+  }
+  const unsigned gx = (unsigned)(((my + TJ - 1) / TJ) * ((mz + TK - 1) / TK));
+  int c = CI;
+  while ((int64_t)gx * ((mx + c - 1) / c) * L.n > kRedGrid) c *= 2;
+  *ci = c;
+  return dim3(gx, (unsigned)((mx + c - 1) / c), (unsigned)L.n);
+}
+
+struct Tile {
+  int j0, k0, i0, i1, tj, tk;
+  bool live;
+};
+// tile of this block for a level whose (j, k) extent is (ny_t, nz_t) and i extent ni
+__device__ __forceinline__ Tile tile_of(int ny_t, int nz_t, int ni, int ci) {
+  Tile t;
+  const int ntk = (nz_t + TK - 1) / TK, ntj = (ny_t + TJ - 1) / TJ;
+  const int b = blockIdx.x;
+  t.j0 = (b / ntk) * TJ;
+  t.k0 = (b % ntk) * TK;
+  t.i0 = blockIdx.y * ci;
+  t.i1 = min(t.i0 + ci, ni);
+  t.live = b < ntj * ntk && t.i0 < ni;
+  t.tj = threadIdx.x / TK;
+  t.tk = threadIdx.x % TK;
+  return t;
+}
+
+// ---- G1: vc = interp(curl p) on the cells of plane i (reads x-edges of plane i, y/z-edges
+// of planes i and i+1)
+template <typename T>
+__global__ __launch_bounds__(256) void k_sg1_t(int64_t ncell, Levels L, int ci, const T* __restrict__ p,
+                                               T* __restrict__ vc) {
+  const Lev& v = L.l[blockIdx.z];
+  const Tile t = tile_of(v.ny, v.nz, v.nx, ci);
+  if (!t.live) return;
+  constexpr int XJ = TJ + 1, XK = TK + 1;       // x-edges: rows j..j+1, k..k+1
+  constexpr int YJ = TJ, YK = TK + 1;           // y-edges: k..k+1
+  constexpr int ZJ = TJ + 1, ZK = TK;           // z-edges: j..j+1
+  __shared__ T sx[XJ * XK];
+  __shared__ T sy[2][YJ * YK];
+  __shared__ T sz[2][ZJ * ZK];
+  const int ny1 = v.ny + 1, nz1 = v.nz + 1;
+  auto stage_x = [&](int i) {
+    for (int q = threadIdx.x; q < XJ * XK; q += blockDim.x) {
+      const int j = t.j0 + q / XK, k = t.k0 + q % XK;
+      const bool in = i < v.nx && j < ny1 && k < nz1;
+      sx[q] = in ? p[v.e0 + ((int64_t)i * ny1 + j) * nz1 + k] : T(0);
+    }
+  };
+  auto stage_y = [&](int i, int slot) {
+    for (int q = threadIdx.x; q < YJ * YK; q += blockDim.x) {
+      const int j = t.j0 + q / YK, k = t.k0 + q % YK;
+      const bool in = i <= v.nx && j < v.ny && k < nz1;
+      sy[slot][q] = in ? p[v.e1 + ((int64_t)i * v.ny + j) * nz1 + k] : T(0);
+    }
+  };
+  auto stage_z = [&](int i, int slot) {
+    for (int q = threadIdx.x; q < ZJ * ZK; q += blockDim.x) {
+      const int j = t.j0 + q / ZK, k = t.k0 + q % ZK;
+      const bool in = i <= v.nx && j < ny1 && k < v.nz;
+      sz[slot][q] = in ? p[v.e2 + ((int64_t)i * ny1 + j) * v.nz + k] : T(0);
+    }
+  };
+  stage_y(t.i0, t.i0 & 1);
+  stage_z(t.i0, t.i0 & 1);
+  const T cf = (T)(1.0 / (2.0 * v.h * v.h));
+  const int j = t.j0 + t.tj, k = t.k0 + t.tk;
+  for (int i = t.i0; i < t.i1; ++i) {
+    const int a = i & 1, b = a ^ 1;             // slot of plane i, plane i+1
+    stage_x(i);
+    stage_y(i + 1, b);
+    stage_z(i + 1, b);
+    __syncthreads();
+    if (j < v.ny && k < v.nz) {
+      auto PX = [&](int dj, int dk) { return sx[(t.tj + dj) * XK + t.tk + dk]; };
+      auto PY = [&](int s, int dk) { return sy[s][t.tj * YK + t.tk + dk]; };
+      auto PZ = [&](int s, int dj) { return sz[s][(t.tj + dj) * ZK + t.tk]; };
+      const T ux0 = (PY(a, 0) - PY(a, 1)) + (PZ(a, 1) - PZ(a, 0));
+      const T ux1 = (PY(b, 0) - PY(b, 1)) + (PZ(b, 1) - PZ(b, 0));
+      const T uy0 = (PX(0, 1) - PX(0, 0)) - (PZ(b, 0) - PZ(a, 0));
+      const T uy1 = (PX(1, 1) - PX(1, 0)) - (PZ(b, 1) - PZ(a, 1));
+      const T uz0 = (PY(b, 0) - PY(a, 0)) - (PX(1, 0) - PX(0, 0));
+      const T uz1 = (PY(b, 1) - PY(a, 1)) - (PX(1, 1) - PX(0, 1));
+      const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
+      vc[c] = (ux0 + ux1) * cf;
+      vc[ncell + c] = (uy0 + uy1) * cf;
+      vc[2 * ncell + c] = (uz0 + uz1) * cf;
+    }
+    __syncthreads();
+  }
+}
+
+// ---- V: y = rou x - (hdt/h^3)(a - b) on the cells of plane i (planes i-1, i, i+1 of x)
+template <typename T>
+__global__ __launch_bounds__(256) void k_sv_t(int64_t ncell, Levels L, int ci, T rou, T visc, T hdt,
+                                              const T* __restrict__ x, T* __restrict__ y) {
+  const Lev& v = L.l[blockIdx.z];
+  const Tile t = tile_of(v.ny, v.nz, v.nx, ci);
+  if (!t.live) return;
+  constexpr int PJ = TJ + 2, PK = TK + 2, PS = PJ * PK;
+  __shared__ T sx[3][3][PS];                    // [plane slot][component][(j, k) with halo]
+  auto stage = [&](int i) {
+    const int slot = (i + 3) % 3;
+    for (int q = threadIdx.x; q < PS; q += blockDim.x) {
+      const int j = t.j0 - 1 + q / PK, k = t.k0 - 1 + q % PK;
+      const bool in = i >= 0 && i < v.nx && j >= 0 && j < v.ny && k >= 0 && k < v.nz;
+      const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) sx[slot][d][q] = in ? x[d * ncell + c] : T(0);
+    }
+  };
+  stage(t.i0 - 1);
+  stage(t.i0);
+  const T kA = (T)v.h * visc;
+  const T hv = hdt * (T)(1.0 / (v.h * v.h * v.h));
+  const int j = t.j0 + t.tj, k = t.k0 + t.tk;
+  const int q = (t.tj + 1) * PK + t.tk + 1;
+  for (int i = t.i0; i < t.i1; ++i) {
+    stage(i + 1);
+    __syncthreads();
+    if (j < v.ny && k < v.nz) {
+      const int sm = (i + 2) % 3, s0 = i % 3, sp = (i + 1) % 3;
+      const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const T xc = sx[s0][d][q];
+        T a = kA * (sx[sp][d][q] - xc) + kA * (sx[s0][d][q + PK] - xc);
+        a = a + kA * (sx[s0][d][q + 1] - xc);
+        T b = kA * (xc - sx[sm][d][q]) + kA * (xc - sx[s0][d][q - PK]);
+        b = b + kA * (xc - sx[s0][d][q - 1]);
+        y[d * ncell + c] = rou * xc - hv * (a - b);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---- G3: w on the x-, y- and z-edges of plane i (planes i-1 and i of y); irregular rows
+// and empty slots get 0 (the composite pass writes the irregular rows), rows on the box's
+// last plane/row are never regular and are left to the composite pass
+template <typename T>
+__global__ __launch_bounds__(256) void k_sg3_t(int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
+                                               const T* __restrict__ y, T* __restrict__ w, const T* __restrict__ p,
+                                               double* __restrict__ partials) {
+  __shared__ double sh[kFinBlock / 64];
+  const Lev& v = L.l[blockIdx.z];
+  const Tile t = tile_of(v.ny, v.nz, v.nx, ci);
+  double dot = 0.0;
+  if (t.live) {
+    constexpr int PJ = TJ + 1, PK = TK + 1, PS = PJ * PK;   // low-side halo (j-1, k-1)
+    __shared__ T sy[2][3][PS];
+    auto stage = [&](int i) {
+      const int slot = (i + 2) & 1;
+      for (int q = threadIdx.x; q < PS; q += blockDim.x) {
+        const int j = t.j0 - 1 + q / PK, k = t.k0 - 1 + q % PK;
+        const bool in = i >= 0 && i < v.nx && j >= 0 && j < v.ny && k >= 0 && k < v.nz;
+        const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) sy[slot][d][q] = in ? y[d * ncell + c] : T(0);
+      }
+    };
+    stage(t.i0 - 1);
+    const T hh = (T)(0.5 * v.h);
+    const int j = t.j0 + t.tj, k = t.k0 + t.tk;
+    const int q = (t.tj + 1) * PK + t.tk + 1;      // cell (i, j, k) in the staged plane
+    const int ny1 = v.ny + 1, nz1 = v.nz + 1;
+    for (int i = t.i0; i < t.i1; ++i) {
+      stage(i);
+      __syncthreads();
+      if (j < v.ny && k < v.nz) {
+        const int s0 = i & 1, sm = s0 ^ 1;          // planes i and i-1
+        auto Y = [&](int s, int d, int dj, int dk) { return sy[s][d][q + dj * PK + dk]; };
+        {   // x-edge (i, j, k): -F_y(i,j,k) + F_y(i,j,k-1) + F_z(i,j,k) - F_z(i,j-1,k)
+          const int64_t e = v.e0 + ((int64_t)i * ny1 + j) * nz1 + k;
+          const T fy0 = hh * (Y(s0, 1, -1, 0) + Y(s0, 1, 0, 0));
+          const T fy1 = hh * (Y(s0, 1, -1, -1) + Y(s0, 1, 0, -1));
+          const T fz0 = hh * (Y(s0, 2, 0, -1) + Y(s0, 2, 0, 0));
+          const T fz1 = hh * (Y(s0, 2, -1, -1) + Y(s0, 2, -1, 0));
+          const T val = reg[e] ? (fy1 - fy0) + (fz0 - fz1) : T(0);
+          w[e] = val;
+          if (p) dot += (double)p[e] * (double)val;
+        }
+        {   // y-edge (i, j, k): F_x(i,j,k) - F_x(i,j,k-1) - F_z(i,j,k) + F_z(i-1,j,k)
+          const int64_t e = v.e1 + ((int64_t)i * v.ny + j) * nz1 + k;
+          const T fx0 = hh * (Y(sm, 0, 0, 0) + Y(s0, 0, 0, 0));
+          const T fx1 = hh * (Y(sm, 0, 0, -1) + Y(s0, 0, 0, -1));
+          const T fz0 = hh * (Y(s0, 2, 0, -1) + Y(s0, 2, 0, 0));
+          const T fz1 = hh * (Y(sm, 2, 0, -1) + Y(sm, 2, 0, 0));
+          const T val = reg[e] ? (fx0 - fx1) + (fz1 - fz0) : T(0);
+          w[e] = val;
+          if (p) dot += (double)p[e] * (double)val;
+        }
+        {   // z-edge (i, j, k): -F_x(i,j,k) + F_x(i,j-1,k) + F_y(i,j,k) - F_y(i-1,j,k)
+          const int64_t e = v.e2 + ((int64_t)i * ny1 + j) * v.nz + k;
+          const T fx0 = hh * (Y(sm, 0, 0, 0) + Y(s0, 0, 0, 0));
+          const T fx1 = hh * (Y(sm, 0, -1, 0) + Y(s0, 0, -1, 0));
+          const T fy0 = hh * (Y(s0, 1, -1, 0) + Y(s0, 1, 0, 0));
+          const T fy1 = hh * (Y(sm, 1, -1, 0) + Y(sm, 1, 0, 0));
+          const T val = reg[e] ? (fx1 - fx0) + (fy0 - fy1) : T(0);
+          w[e] = val;
+          if (p) dot += (double)p[e] * (double)val;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (partials) {
+    double r = block_sum(dot, sh);
+    if (threadIdx.x == 0) partials[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = r;
   }
 }
 
@@ -345,6 +425,13 @@ __global__ void k_gcg_dir(int64_t n, const double* __restrict__ scal, const T* _
     p[i] = r[i] * prec[i] + beta * p[i];
 }
 __global__ void k_gcg_shift(double* __restrict__ scal) { scal[S_ETA] = scal[S_ETA_NEW]; }
+// residual replacement: eta_new <- the replaced residual's eta, beta from it
+__global__ void k_gcg_replace(const double* __restrict__ eta_src, double* __restrict__ scal) {
+  const double en = eta_src[0];
+  const double eta = scal[S_ETA];
+  scal[S_ETA_NEW] = en;
+  scal[S_BETA] = en / (eta != 0.0 ? eta : __builtin_inf());
+}
 
 template <typename T>
 __global__ void k_gcg_begin(int64_t n, const T* __restrict__ res, const T* __restrict__ s,
@@ -413,30 +500,30 @@ __global__ void k_permute(int64_t n, const int64_t* __restrict__ idx, const int8
 // ------------------------------------------------------- launchers
 inline int slice_grid(int64_t nslice) { return grid_for(nslice * 64); }
 inline int slice_red_grid(int64_t nslice) { return red_grid_for(nslice * 64); }
-
-// w = A p on the grid; the p.w partials go to ws->partials [0, nb1) and [kRedGrid, kRedGrid+nb2)
 template <typename T>
 int gflow(const pyib_grid* g, const pyib_work* ws, double rou, double visc, double hdt, const T* p, T* w,
           bool want_dot, int* nb1, int* nb2, hipStream_t st) {
   const Levels L = levels_of(g);
   T* ca = static_cast<T*>(ws->cell_a);
   T* cb = static_cast<T*>(ws->cell_b);
-  k_sg1<T><<<grid_for(g->ncell), kBlock, 0, st>>>(g->ncell, L, g->cell_reg, p, ca);
+  int ci = CI;
+  const dim3 gt = tile_grid(L, &ci);
+  k_sg1_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, p, ca);
   if (g->g1_nslice > 0)
     k_sell_g1<T><<<slice_grid(g->g1_nslice), kBlock, 0, st>>>(g->g1_nslice, g->ncell, g->g1_row, g->g1_width,
                                                              g->g1_sptr, g->g1_col, (const T*)g->g1_coef, p, ca);
-  k_sv<T><<<grid_for(g->ncell), kBlock, 0, st>>>(g->ncell, L, g->cell_reg, (T)rou, (T)visc, (T)hdt, ca, cb);
+  k_sv_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, (T)rou, (T)visc, (T)hdt, ca, cb);
   if (g->v_nslice > 0)
     k_sell_v<T><<<slice_grid(g->v_nslice), kBlock, 0, st>>>(g->v_nslice, g->ncell, g->v_row, g->v_width, g->v_sptr,
                                                            g->v_col, (const T*)g->v_coef, (T)rou, (T)(hdt * visc),
                                                            ca, cb);
-  const int n1 = slice_red_grid(g->g3_nslice), n2 = red_grid_for(g->nedge);
+  const int n1 = slice_red_grid(g->g3_nslice), n2 = (int)(gt.x * gt.y * gt.z);
+  k_sg3_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, g->edge_reg, cb, w, want_dot ? p : nullptr,
+                                    want_dot ? ws->partials + kRedGrid : nullptr);
   if (g->g3_nslice > 0)
     k_sell_g3<T><<<n1, kBlock, 0, st>>>(g->g3_nslice, g->ncell, g->g3_row, g->g3_width, g->g3_sptr, g->g3_col,
                                         (const T*)g->g3_coef, g->dir, cb, w, want_dot ? p : nullptr,
                                         want_dot ? ws->partials : nullptr);
-  k_sg3<T><<<n2, kBlock, 0, st>>>(g->nedge, g->ncell, L, g->edge_reg, cb, w, want_dot ? p : nullptr,
-                                  want_dot ? ws->partials + kRedGrid : nullptr);
   *nb1 = g->g3_nslice > 0 ? n1 : 0;
   *nb2 = n2;
   return 0;
@@ -544,6 +631,11 @@ int pyib_grid_cg_true_residual(const pyib_grid* g, const pyib_work* ws, const py
   }
   k_finalize<<<1, kFinBlock, 0, st>>>(ws->partials, nb, 2, out2);
   return check_launch("pyib_grid_cg_true_residual");
+}
+
+int pyib_cg_replace_residual(double* scal, const double* eta, void* stream) {
+  k_gcg_replace<<<1, 1, 0, S(stream)>>>(eta, scal);
+  return check_launch("pyib_cg_replace_residual");
 }
 
 int pyib_permute(int dtype_in, const void* in, int dtype_out, void* out, int64_t n_out, const int64_t* idx,

@@ -34,7 +34,7 @@ class HipDriver:
 
     def __init__(self, core, mesh, visc, rou=1.0, ac=0.5, ad=0.5, *, cg_max_iterations=1000,
                  fail_on_cg_nonconvergence=False, cg_check_interval=20, precision="float64",
-                 mixed_inner_tol=1e-4, mixed_max_outer=50, structured=True, layout=None):
+                 mixed_inner_tol=0.1, mixed_max_outer=1000, structured=True, layout=None):
         self.core = core
         self.grid = None
         if structured:
@@ -327,7 +327,8 @@ class HipDriver:
         return self._x.clone(), int(m)
 
     def _helm_solve_mixed(self, res_e, s, dt, tol1, tol2, cg_max):
-        """Defect correction: x <- x + A32^-1 r (float32 PCG), r = res_e - A64 (x - s) in float64."""
+        """Defect correction: float32 PCG on A32 d = r with float64 reliable updates
+        (x += d in float64, r <- res_e - A64 (x - s) in float64, search direction kept)."""
         core = self.core
         m64, w64 = core.struct("float64"), core.work_struct("float64")
         m32, w32 = core.struct("float32"), core.work_struct("float32")
@@ -343,22 +344,21 @@ class HipDriver:
             o = self._out2.cpu().numpy()
             self._finish(0, float(o[0]), l2_init, l2_init, err_tot, cg_max)
             return self._x.clone(), 0
-        # _x holds s, _r the masked float64 residual
+        # _x holds s, _r the masked float64 residual; float32 PCG with reliable updates (as
+        # in _helm_solve_grid)
         total = 0
         outer = 0
         eta_f, l2_f = eta0, l2_init
         zero32 = self._z32
         zero32.zero_()
+        N.call("pyib_convert", N.F64, ptr(self._r), N.F32, ptr(self._r32), n, st())
+        N.call("pyib_cg_begin", m32, w32, C.byref(self._cg32), ptr(self._r32), ptr(zero32), 0.0,
+               self.mixed_inner_tol, st())
+        inner_err = float(self._scal32[N.S_ERR].item())
         while True:
             outer += 1
-            N.call("pyib_convert", N.F64, ptr(self._r), N.F32, ptr(self._r32), n, st())
-            # inner float32 PCG from 0: stop at eta <= tol_in^2 eta_inner0 (tested at the cadence)
-            N.call("pyib_cg_begin", m32, w32, C.byref(self._cg32), ptr(self._r32), ptr(zero32), 0.0,
-                   self.mixed_inner_tol, st())
-            s32 = self._scal32.cpu().numpy()
-            inner_err = float(s32[N.S_ERR])
-            budget = max(cg_max - total, 1)
-            k = self._pcg_loop(m32, w32, self._cg32, self._scal32, budget, lambda: inner_err, dt)
+            k = self._pcg_loop(m32, w32, self._cg32, self._scal32, max(cg_max - total, 1),
+                               lambda: max(inner_err, err_tot), dt)
             total += k
             N.call("pyib_add_into_f64", N.F32, ptr(self._x32), ptr(self._x), n, st())
             N.call("pyib_cg_true_residual", m64, w64, C.byref(self._cg64), ptr(res_e), ptr(s), ptr(self._tmp),
@@ -367,7 +367,11 @@ class HipDriver:
             eta_f, l2_f = float(o[0]), float(o[1])
             if not np.isfinite(eta_f) or eta_f <= err_tot or total >= cg_max or outer >= self.mixed_max_outer:
                 break
-            self._r.copy_(self._tr)
+            self._x32.zero_()
+            N.call("pyib_convert", N.F64, ptr(self._tr), N.F32, ptr(self._r32), n, st())
+            N.call("pyib_cg_replace_residual", ptr(self._scal32), ptr(self._out2), st())
+            N.call("pyib_cg_direction", m32, w32, C.byref(self._cg32), st())
+            inner_err = (self.mixed_inner_tol ** 2) * eta_f
         self.helmholtz_outer_last_step += outer
         self._finish(total, eta_f, l2_f, l2_init, err_tot, cg_max)
         return self._x.clone(), int(total)
@@ -396,18 +400,22 @@ class HipDriver:
             o = self._out2.cpu().numpy()
             self._finish(m, float(o[0]), float(o[1]), l2_init, err_tot, cg_max)
             return g.to_reference(self._xi), int(m)
+        # float32 PCG with reliable updates: every time the float32 recurrence residual has
+        # dropped by mixed_inner_tol (or reached the float64 target) the float32 correction is
+        # added into the float64 solution, the float64 true residual is formed and it replaces
+        # the float32 residual; the search direction is kept, so the Krylov space is not lost.
         m32, w32 = g.struct("float32"), g.work_struct("float32")
         total, outer = 0, 0
         eta_f, l2_f = eta0, l2_init
         self._zi32.zero_()
+        N.call("pyib_convert", N.F64, ptr(self._ri), N.F32, ptr(self._ri32), n, st())
+        N.call("pyib_grid_cg_begin", m32, w32, C.byref(self._cgi32), ptr(self._ri32), ptr(self._zi32), 0.0,
+               self.mixed_inner_tol, st())
+        inner_err = float(self._scal32[N.S_ERR].item())
         while True:
             outer += 1
-            N.call("pyib_convert", N.F64, ptr(self._ri), N.F32, ptr(self._ri32), n, st())
-            N.call("pyib_grid_cg_begin", m32, w32, C.byref(self._cgi32), ptr(self._ri32), ptr(self._zi32), 0.0,
-                   self.mixed_inner_tol, st())
-            inner_err = float(self._scal32[N.S_ERR].item())
-            k = self._pcg_loop(m32, w32, self._cgi32, self._scal32, max(cg_max - total, 1), lambda: inner_err, dt,
-                               grid=True)
+            k = self._pcg_loop(m32, w32, self._cgi32, self._scal32, max(cg_max - total, 1),
+                               lambda: max(inner_err, err_tot), dt, grid=True)
             total += k
             N.call("pyib_add_into_f64", N.F32, ptr(self._xi32), ptr(self._xi), n, st())
             N.call("pyib_grid_cg_true_residual", m64, w64, C.byref(self._cgi), ptr(self._resi), ptr(self._si),
@@ -416,7 +424,11 @@ class HipDriver:
             eta_f, l2_f = float(o[0]), float(o[1])
             if not np.isfinite(eta_f) or eta_f <= err_tot or total >= cg_max or outer >= self.mixed_max_outer:
                 break
-            self._ri.copy_(self._tri)
+            self._xi32.zero_()
+            N.call("pyib_convert", N.F64, ptr(self._tri), N.F32, ptr(self._ri32), n, st())
+            N.call("pyib_cg_replace_residual", ptr(self._scal32), ptr(self._out2), st())
+            N.call("pyib_grid_cg_direction", m32, C.byref(self._cgi32), st())
+            inner_err = (self.mixed_inner_tol ** 2) * eta_f
         self.helmholtz_outer_last_step += outer
         self._finish(total, eta_f, l2_f, l2_init, err_tot, cg_max)
         return g.to_reference(self._xi), int(total)

@@ -65,8 +65,9 @@ class StructuredLayout:
     edge_sigma: np.ndarray      # +-1
     edge_inv: np.ndarray        # internal -> reference edge (-1: empty slot)
     sigma_int: np.ndarray       # +-1 on internal slots (1 on empty)
-    reg_cell: np.ndarray        # uint8 over internal cells
-    reg_edge: np.ndarray        # uint8 over internal edges
+    reg_cell: np.ndarray        # uint8 over internal cells: V stencil row
+    reg_edge: np.ndarray        # uint8 over internal edges: G3 stencil row
+    reg_g1: np.ndarray = None   # uint8 over internal cells: G1 stencil row
     g1: Composite = None
     vis: Composite = None
     g3: Composite = None
@@ -212,17 +213,23 @@ def _close(a, b, tol, scale):
 
 
 def _verify_regular(lay, mesh, geo, tables, tol, demote_neighbours=True):
-    """Demote regular rows whose surrounding geometry differs from the closed form."""
+    """Regularity per pass, from the mesh geometry (each pass only needs its own data):
+
+    * G1 (cell <- edges): closed-form vol_ci, exactly six faces, and on each face this
+      cell's rf (rf1 as first owner, rf2 as second) = +-h/2 e_d with four links;
+    * V (cell <- cells): closed-form vol_ci and six internal same-level faces with the full
+      closed form (rf1, rf2, AoL = h, Af = h^2, four links), at least ``margin`` cells from
+      the box and the hole;
+    * G3 (edge <- cells): an edge off the box boundary whose four adjacent cells are real
+      and whose four faces (exactly four rot links) have closed-form rf1 and rf2.
+    """
     ncell = geo.ncell
-    cinv = np.full(lay.ncell, -1, np.int64)
-    cinv[lay.cell_map] = np.arange(ncell)
     lev_of_cell = np.empty(ncell, np.int64)
     for L, lv in enumerate(lay.levels):
         sel = (lay.cell_map >= lv.cell_off) & (lay.cell_map < lv.cell_off + lv.ncell)
         lev_of_cell[sel] = L
     h = np.array([lv.h for lv in lay.levels])[lev_of_cell]
-    ok_cell = _close(geo.vol_ci, 1.0 / h ** 3, tol, 1.0 / h ** 3)
-    # faces: both-owner cells (or the one owner) must see +-h/2 e_d, AoL = h, Af = h^2, 4 +-1 links
+    ok_vol = _close(geo.vol_ci, 1.0 / h ** 3, tol, 1.0 / h ** 3)
     F2C = geo.F2C
     hf = h[F2C[:, 0]]
     dn = np.argmax(np.abs(geo.nf), axis=1)
@@ -230,40 +237,58 @@ def _verify_regular(lay, mesh, geo, tables, tol, demote_neighbours=True):
     e_d[np.arange(geo.nface), dn] = 1.0
     half = 0.5 * hf[:, None] * e_d
     sgn = np.sign(geo.rf1[np.arange(geo.nface), dn])
-    ok_face = np.all(_close(np.abs(geo.rf1), half, tol, hf[:, None]), axis=1)
-    ok_face &= np.all(_close(np.abs(geo.rf2), np.where(geo.m1[:, None], half, 0.0), tol, hf[:, None]), axis=1)
-    ok_face &= _close(np.abs(geo.AoL), hf, tol, hf) & _close(geo.Af, hf * hf, tol, hf * hf)
-    ok_face &= geo.m1 & (h[np.maximum(F2C[:, 1], 0)] == hf) & (sgn != 0)
-    ok_face &= np.diff(tables.curl_ptr) == 4
-    bad_face = ~ok_face
-    bad_cell = ~ok_cell.copy()
-    bad_cell[F2C[bad_face, 0]] = True
-    m1b = bad_face & geo.m1
-    bad_cell[F2C[m1b, 1]] = True
-    # a regular cell needs good own geometry and good neighbours (V reads them)
-    demote = np.zeros(lay.ncell, bool)
-    demote[lay.cell_map[bad_cell]] = True
-    nb_bad = np.zeros(ncell, bool)
-    own = np.repeat(np.arange(ncell), np.diff(tables.cf_ptr))
-    oth = tables.cf_other
-    inner = oth < ncell
-    nb_bad_idx = own[inner][bad_cell[oth[inner]]]
-    nb_bad[nb_bad_idx] = True
+    links4 = np.diff(tables.curl_ptr) == 4
+    same_h = geo.m1 & (h[np.maximum(F2C[:, 1], 0)] == hf)
+    rf1_ok = np.all(_close(np.abs(geo.rf1), half, tol, hf[:, None]), axis=1) & (sgn != 0)
+    rf2_ok = same_h & np.all(_close(np.abs(geo.rf2), half, tol, hf[:, None]), axis=1)
+    ok_face = rf1_ok & rf2_ok & links4 & _close(np.abs(geo.AoL), hf, tol, hf) & _close(geo.Af, hf * hf, tol, hf * hf)
+    # per (cell, face) entries of the cell-face table
+    cnt = np.diff(tables.cf_ptr)
+    own = np.repeat(np.arange(ncell), cnt)
+    pos = np.arange(tables.cf_face.shape[0]) - np.repeat(tables.cf_ptr[:-1], cnt)
+    first = pos < np.repeat(tables.cf_mid - tables.cf_ptr[:-1], cnt)
+    f = tables.cf_face
+    g1_ent = np.where(first, rf1_ok[f], rf2_ok[f]) & links4[f]
+    g1_ok = ok_vol & (cnt == 6) & (np.bincount(own, weights=(~g1_ent).astype(np.float64), minlength=ncell) == 0)
+    v_ok = ok_vol & (cnt == 6) & (np.bincount(own, weights=(~ok_face[f]).astype(np.float64), minlength=ncell) == 0)
+    real = np.zeros(lay.ncell, bool)
+    real[lay.cell_map] = True
+    reg_g1 = np.zeros(lay.ncell, bool)
+    reg_g1[lay.cell_map[g1_ok]] = True
+    reg_v = lay.reg_cell.astype(bool)           # structural (margin) part from _classify
+    v_geo = np.zeros(lay.ncell, bool)
+    v_geo[lay.cell_map[v_ok]] = True
+    reg_v &= v_geo & real
     if demote_neighbours:
-        demote[lay.cell_map[nb_bad]] = True
-    before = int(lay.reg_cell.sum())
-    lay.reg_cell[demote] = 0
-    # edges: re-derive from the (possibly demoted) cells
+        bad = real & ~reg_v
+        nb = np.zeros(ncell, bool)
+        inner = tables.cf_other < ncell
+        cinv = np.full(lay.ncell, -1, np.int64)
+        cinv[lay.cell_map] = np.arange(ncell)
+        badr = bad[lay.cell_map]
+        nb[own[inner][badr[tables.cf_other[inner]]]] = True
+        reg_v[lay.cell_map[nb]] = False
+    lay.reg_g1 = reg_g1.astype(np.uint8)
+    lay.reg_cell = reg_v.astype(np.uint8)
+    # G3: structural (off the box boundary, four real adjacent cells) and geometric
+    ecnt = np.diff(tables.rot_ptr)
+    eown = np.repeat(np.arange(geo.nedge), ecnt)
+    fidx = tables.rot_idx.astype(np.int64)
+    fidx = np.where(fidx >= 0, fidx, -fidx - 1)
+    f_g3 = rf1_ok & rf2_ok & links4
+    g3_ok = (ecnt == 4) & (np.bincount(eown, weights=(~f_g3[fidx]).astype(np.float64), minlength=geo.nedge) == 0)
+    reg_e = np.zeros(lay.nedge, bool)
+    reg_e[lay.edge_map[g3_ok]] = True
     for lv in lay.levels:
-        rc = lay.reg_cell[lv.cell_off:lv.cell_off + lv.ncell].reshape(lv.nx, lv.ny, lv.nz).astype(bool)
+        rc = real[lv.cell_off:lv.cell_off + lv.ncell].reshape(lv.nx, lv.ny, lv.nz)
         for d in range(3):
-            re = _reg_edges_level(lv, rc, d)
-            seg = slice(lv.e_off[d], lv.e_off[d] + re.size)
-            lay.reg_edge[seg] &= re.ravel().astype(np.uint8)
-    # an edge slot with no mesh edge can never be regular
-    lay.reg_edge[lay.edge_inv < 0] = 0
-    lay.stats.update(regular_cells=int(lay.reg_cell.sum()), demoted_cells=before - int(lay.reg_cell.sum()),
-                     regular_edges=int(lay.reg_edge.sum()))
+            st = _reg_edges_level(lv, rc, d)        # the four adjacent cells exist (real, in the box)
+            seg = slice(lv.e_off[d], lv.e_off[d] + st.size)
+            reg_e[seg] &= st.ravel()
+    reg_e[lay.edge_inv < 0] = False
+    lay.reg_edge = reg_e.astype(np.uint8)
+    lay.stats.update(regular_cells=int(reg_v.sum()), regular_g1_cells=int(reg_g1.sum()),
+                     regular_edges=int(reg_e.sum()))
 
 
 def expand_ranges(starts, counts):
@@ -301,7 +326,9 @@ def _composites(lay, geo, t):
     cmap, emap, sig = lay.cell_map, lay.edge_map, lay.edge_sigma.astype(np.float64)
     real_cell = np.zeros(lay.ncell, bool)
     real_cell[cmap] = True
-    irr_cells_int = np.flatnonzero(real_cell & (lay.reg_cell == 0))
+    irr_g1_int = np.flatnonzero(real_cell & (lay.reg_g1 == 0))
+    irr_v_int = np.flatnonzero(real_cell & (lay.reg_cell == 0))
+    irr_cells_int = np.union1d(irr_g1_int, irr_v_int)
     cinv = np.full(lay.ncell, -1, np.int64)
     cinv[cmap] = np.arange(ncell)
     irr_ref = cinv[irr_cells_int]
@@ -324,16 +351,16 @@ def _composites(lay, geo, t):
     e_ref = np.where(sv >= 0, sv, -sv - 1)
     s_link = np.where(sv >= 0, 1.0, -1.0)
     coef = rfc[rep] * (s_link * sig[e_ref])[:, None]
-    ptr, col, cf = _csr_rows(cmap[own_e][rep], lay.ncell, emap[e_ref], coef, irr_cells_int)
-    lay.g1 = Composite(irr_cells_int.astype(np.int32), ptr, col, cf)
+    ptr, col, cf = _csr_rows(cmap[own_e][rep], lay.ncell, emap[e_ref], coef, irr_g1_int)
+    lay.g1 = Composite(irr_g1_int.astype(np.int32), ptr, col, cf)
     # --- V (cell <- cell): off-diagonal vol_ci*AoL, diagonal -vol_ci*sum AoL
     aol = geo.AoL[f_e] * geo.vol_ci[own_e]
     inner = oth_e < ncell
     rows = np.concatenate([cmap[own_e][inner], cmap[own_e]])
     cols = np.concatenate([cmap[oth_e[inner]], cmap[own_e]])
     vals = np.concatenate([aol[inner], -aol])
-    ptr, col, cf = _csr_rows(rows, lay.ncell, cols, vals, irr_cells_int)
-    lay.vis = Composite(irr_cells_int.astype(np.int32), ptr, col, cf)
+    ptr, col, cf = _csr_rows(rows, lay.ncell, cols, vals, irr_v_int)
+    lay.vis = Composite(irr_v_int.astype(np.int32), ptr, col, cf)
     # --- G3: edge <- cell, for real irregular edges
     real_edge = lay.edge_inv >= 0
     irr_edges_int = np.flatnonzero(real_edge & (lay.reg_edge == 0))
@@ -350,7 +377,8 @@ def _composites(lay, geo, t):
     vals = np.concatenate([geo.rf1[f_ref] * s_link[:, None], -geo.rf2[f_ref][c2 >= 0] * s_link[c2 >= 0][:, None]])
     ptr, col, cf = _csr_rows(rows, lay.nedge, cols, vals, irr_edges_int)
     lay.g3 = Composite(irr_edges_int.astype(np.int32), ptr, col, cf)
-    lay.stats.update(irregular_cells=int(irr_cells_int.shape[0]), irregular_edges=int(irr_edges_int.shape[0]),
+    lay.stats.update(irregular_cells=int(irr_v_int.shape[0]), irregular_g1_cells=int(irr_g1_int.shape[0]),
+                     irregular_edges=int(irr_edges_int.shape[0]),
                      g1_nnz=int(lay.g1.col.shape[0]), v_nnz=int(lay.vis.col.shape[0]), g3_nnz=int(lay.g3.col.shape[0]))
 
 

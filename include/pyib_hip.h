@@ -75,7 +75,7 @@ typedef struct pyib_work {
   double *partials;         /* >= PYIB_MAX_PARTIALS doubles */
 } pyib_work;
 
-#define PYIB_MAX_PARTIALS 65536
+#define PYIB_MAX_PARTIALS 8192
 
 /* CG state; vectors have the mesh dtype.  scal is a device array of
  * PYIB_NSCAL doubles: [eta, eta_new, den, alpha, beta, rr, err_tot, l2_init]. */
@@ -218,6 +218,11 @@ int pyib_interior_momentum_adjoint(int64_t n, const int32_t *idx, const double *
                                    double *gvc, double *partials, double *gcenter3, void *stream);
 
 /* ---- mixed precision helpers ---- */
+/* Residual replacement in a CG state stopped at a convergence check (direction not yet
+ * updated): scal[eta_new] <- *eta (device), scal[beta] <- *eta / scal[eta].  The caller has
+ * written the replaced residual into cg->r; the next pyib_*cg_direction continues the CG
+ * with the kept search direction.  No reference counterpart (mixed precision is additive). */
+int pyib_cg_replace_residual(double *scal, const double *eta, void *stream);
 int pyib_convert(int dtype_in, const void *in, int dtype_out, void *out, int64_t n, void *stream);
 /* y[i] += a[i] (a in dtype_a, y double) */
 int pyib_add_into_f64(int dtype_a, const void *a, double *y, int64_t n, void *stream);

@@ -33,7 +33,7 @@ def emulate(lay, p, rou, visc, hdt):
         c = 1.0 / (2.0 * lv.h * lv.h)
         v = np.stack([(Ux[:-1] + Ux[1:]) * c, (Uy[:, :-1] + Uy[:, 1:]) * c, (Uz[:, :, :-1] + Uz[:, :, 1:]) * c])
         sl = slice(lv.cell_off, lv.cell_off + lv.ncell)
-        reg = lay.reg_cell[sl].astype(bool)
+        reg = lay.reg_g1[sl].astype(bool)
         for d in range(3):
             vc[d, sl][reg] = v[d].ravel()[reg]
     g1 = lay.g1
