@@ -171,6 +171,14 @@ __device__ __forceinline__ Tile tile_of(int ny_t, int nz_t, int ni, int ci) {
   return t;
 }
 
+// Staging with a one-plane register prefetch: the loads of the plane needed next are
+// issued before the current plane is computed, so their latency overlaps the compute.
+// Each thread stages entries q = tid + r*256 (r < R) of a plane of `size` entries.
+template <typename T, int R>
+struct Prefetch {
+  T v[R];
+};
+
 // ---- G1: vc = interp(curl p) on the cells of plane i (reads x-edges of plane i, y/z-edges
 // of planes i and i+1)
 template <typename T>
@@ -182,43 +190,65 @@ __global__ __launch_bounds__(256) void k_sg1_t(int64_t ncell, Levels L, int ci, 
   constexpr int XJ = TJ + 1, XK = TK + 1;       // x-edges: rows j..j+1, k..k+1
   constexpr int YJ = TJ, YK = TK + 1;           // y-edges: k..k+1
   constexpr int ZJ = TJ + 1, ZK = TK;           // z-edges: j..j+1
-  __shared__ T sx[XJ * XK];
+  constexpr int R = 2;                          // every staged plane has < 2*256 entries
+  __shared__ T sx[2][XJ * XK];
   __shared__ T sy[2][YJ * YK];
   __shared__ T sz[2][ZJ * ZK];
   const int ny1 = v.ny + 1, nz1 = v.nz + 1;
-  auto stage_x = [&](int i) {
-    for (int q = threadIdx.x; q < XJ * XK; q += blockDim.x) {
-      const int j = t.j0 + q / XK, k = t.k0 + q % XK;
-      const bool in = i < v.nx && j < ny1 && k < nz1;
-      sx[q] = in ? p[v.e0 + ((int64_t)i * ny1 + j) * nz1 + k] : T(0);
+  // loads of step i: x-edges of plane i, y/z-edges of plane i+1
+  auto fetch = [&](int i, Prefetch<T, R>& fx, Prefetch<T, R>& fy, Prefetch<T, R>& fz) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int q = threadIdx.x + r * 256;
+      {
+        const int j = t.j0 + q / XK, k = t.k0 + q % XK;
+        const bool in = q < XJ * XK && i < v.nx && j < ny1 && k < nz1;
+        fx.v[r] = in ? p[v.e0 + ((int64_t)i * ny1 + j) * nz1 + k] : T(0);
+      }
+      {
+        const int j = t.j0 + q / YK, k = t.k0 + q % YK;
+        const bool in = q < YJ * YK && i + 1 <= v.nx && j < v.ny && k < nz1;
+        fy.v[r] = in ? p[v.e1 + ((int64_t)(i + 1) * v.ny + j) * nz1 + k] : T(0);
+      }
+      {
+        const int j = t.j0 + q / ZK, k = t.k0 + q % ZK;
+        const bool in = q < ZJ * ZK && i + 1 <= v.nx && j < ny1 && k < v.nz;
+        fz.v[r] = in ? p[v.e2 + ((int64_t)(i + 1) * ny1 + j) * v.nz + k] : T(0);
+      }
     }
   };
-  auto stage_y = [&](int i, int slot) {
-    for (int q = threadIdx.x; q < YJ * YK; q += blockDim.x) {
-      const int j = t.j0 + q / YK, k = t.k0 + q % YK;
-      const bool in = i <= v.nx && j < v.ny && k < nz1;
-      sy[slot][q] = in ? p[v.e1 + ((int64_t)i * v.ny + j) * nz1 + k] : T(0);
+  auto store = [&](int i, const Prefetch<T, R>& fx, const Prefetch<T, R>& fy, const Prefetch<T, R>& fz) {
+    const int a = i & 1, b = a ^ 1;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int q = threadIdx.x + r * 256;
+      if (q < XJ * XK) sx[a][q] = fx.v[r];
+      if (q < YJ * YK) sy[b][q] = fy.v[r];
+      if (q < ZJ * ZK) sz[b][q] = fz.v[r];
     }
   };
-  auto stage_z = [&](int i, int slot) {
-    for (int q = threadIdx.x; q < ZJ * ZK; q += blockDim.x) {
-      const int j = t.j0 + q / ZK, k = t.k0 + q % ZK;
-      const bool in = i <= v.nx && j < ny1 && k < v.nz;
-      sz[slot][q] = in ? p[v.e2 + ((int64_t)i * ny1 + j) * v.nz + k] : T(0);
-    }
-  };
-  stage_y(t.i0, t.i0 & 1);
-  stage_z(t.i0, t.i0 & 1);
+  // y/z-edges of plane i0 (step i0 reads them from slot i0 & 1)
+  for (int q = threadIdx.x; q < YJ * YK; q += blockDim.x) {
+    const int j = t.j0 + q / YK, k = t.k0 + q % YK;
+    const bool in = t.i0 <= v.nx && j < v.ny && k < nz1;
+    sy[t.i0 & 1][q] = in ? p[v.e1 + ((int64_t)t.i0 * v.ny + j) * nz1 + k] : T(0);
+  }
+  for (int q = threadIdx.x; q < ZJ * ZK; q += blockDim.x) {
+    const int j = t.j0 + q / ZK, k = t.k0 + q % ZK;
+    const bool in = t.i0 <= v.nx && j < ny1 && k < v.nz;
+    sz[t.i0 & 1][q] = in ? p[v.e2 + ((int64_t)t.i0 * ny1 + j) * v.nz + k] : T(0);
+  }
+  Prefetch<T, R> fx, fy, fz;
+  fetch(t.i0, fx, fy, fz);
   const T cf = (T)(1.0 / (2.0 * v.h * v.h));
   const int j = t.j0 + t.tj, k = t.k0 + t.tk;
   for (int i = t.i0; i < t.i1; ++i) {
     const int a = i & 1, b = a ^ 1;             // slot of plane i, plane i+1
-    stage_x(i);
-    stage_y(i + 1, b);
-    stage_z(i + 1, b);
+    store(i, fx, fy, fz);
     __syncthreads();
+    if (i + 1 < t.i1) fetch(i + 1, fx, fy, fz); // in flight while plane i is computed
     if (j < v.ny && k < v.nz) {
-      auto PX = [&](int dj, int dk) { return sx[(t.tj + dj) * XK + t.tk + dk]; };
+      auto PX = [&](int dj, int dk) { return sx[a][(t.tj + dj) * XK + t.tk + dk]; };
       auto PY = [&](int s, int dk) { return sy[s][t.tj * YK + t.tk + dk]; };
       auto PZ = [&](int s, int dj) { return sz[s][(t.tj + dj) * ZK + t.tk]; };
       const T ux0 = (PY(a, 0) - PY(a, 1)) + (PZ(a, 1) - PZ(a, 0));
@@ -243,27 +273,44 @@ __global__ __launch_bounds__(256) void k_sv_t(int64_t ncell, Levels L, int ci, T
   const Lev& v = L.l[blockIdx.z];
   const Tile t = tile_of(v.ny, v.nz, v.nx, ci);
   if (!t.live) return;
-  constexpr int PJ = TJ + 2, PK = TK + 2, PS = PJ * PK;
+  constexpr int PJ = TJ + 2, PK = TK + 2, PS = PJ * PK, R = 2;
   __shared__ T sx[3][3][PS];                    // [plane slot][component][(j, k) with halo]
-  auto stage = [&](int i) {
-    const int slot = (i + 3) % 3;
-    for (int q = threadIdx.x; q < PS; q += blockDim.x) {
+  auto fetch = [&](int i, Prefetch<T, 3 * R>& f) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int q = threadIdx.x + r * 256;
       const int j = t.j0 - 1 + q / PK, k = t.k0 - 1 + q % PK;
-      const bool in = i >= 0 && i < v.nx && j >= 0 && j < v.ny && k >= 0 && k < v.nz;
+      const bool in = q < PS && i >= 0 && i < v.nx && j >= 0 && j < v.ny && k >= 0 && k < v.nz;
       const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) sx[slot][d][q] = in ? x[d * ncell + c] : T(0);
+      for (int d = 0; d < 3; ++d) f.v[3 * r + d] = in ? x[d * ncell + c] : T(0);
     }
   };
-  stage(t.i0 - 1);
-  stage(t.i0);
+  auto store = [&](int i, const Prefetch<T, 3 * R>& f) {
+    const int slot = (i + 3) % 3;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int q = threadIdx.x + r * 256;
+      if (q < PS) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) sx[slot][d][q] = f.v[3 * r + d];
+      }
+    }
+  };
+  Prefetch<T, 3 * R> f;
+  fetch(t.i0 - 1, f);
+  store(t.i0 - 1, f);
+  fetch(t.i0, f);
+  store(t.i0, f);
+  fetch(t.i0 + 1, f);
   const T kA = (T)v.h * visc;
   const T hv = hdt * (T)(1.0 / (v.h * v.h * v.h));
   const int j = t.j0 + t.tj, k = t.k0 + t.tk;
   const int q = (t.tj + 1) * PK + t.tk + 1;
   for (int i = t.i0; i < t.i1; ++i) {
-    stage(i + 1);
+    store(i + 1, f);
     __syncthreads();
+    if (i + 1 < t.i1) fetch(i + 2, f);          // in flight while plane i is computed
     if (j < v.ny && k < v.nz) {
       const int sm = (i + 2) % 3, s0 = i % 3, sp = (i + 1) % 3;
       const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
@@ -293,26 +340,42 @@ __global__ __launch_bounds__(256) void k_sg3_t(int64_t ncell, Levels L, int ci, 
   const Tile t = tile_of(v.ny, v.nz, v.nx, ci);
   double dot = 0.0;
   if (t.live) {
-    constexpr int PJ = TJ + 1, PK = TK + 1, PS = PJ * PK;   // low-side halo (j-1, k-1)
+    constexpr int PJ = TJ + 1, PK = TK + 1, PS = PJ * PK, R = 2;   // low-side halo (j-1, k-1)
     __shared__ T sy[2][3][PS];
-    auto stage = [&](int i) {
-      const int slot = (i + 2) & 1;
-      for (int q = threadIdx.x; q < PS; q += blockDim.x) {
+    auto fetch = [&](int i, Prefetch<T, 3 * R>& f) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int q = threadIdx.x + r * 256;
         const int j = t.j0 - 1 + q / PK, k = t.k0 - 1 + q % PK;
-        const bool in = i >= 0 && i < v.nx && j >= 0 && j < v.ny && k >= 0 && k < v.nz;
+        const bool in = q < PS && i >= 0 && i < v.nx && j >= 0 && j < v.ny && k >= 0 && k < v.nz;
         const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) sy[slot][d][q] = in ? y[d * ncell + c] : T(0);
+        for (int d = 0; d < 3; ++d) f.v[3 * r + d] = in ? y[d * ncell + c] : T(0);
       }
     };
-    stage(t.i0 - 1);
+    auto store = [&](int i, const Prefetch<T, 3 * R>& f) {
+      const int slot = (i + 2) & 1;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int q = threadIdx.x + r * 256;
+        if (q < PS) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) sy[slot][d][q] = f.v[3 * r + d];
+        }
+      }
+    };
+    Prefetch<T, 3 * R> f;
+    fetch(t.i0 - 1, f);
+    store(t.i0 - 1, f);
+    fetch(t.i0, f);
     const T hh = (T)(0.5 * v.h);
     const int j = t.j0 + t.tj, k = t.k0 + t.tk;
     const int q = (t.tj + 1) * PK + t.tk + 1;      // cell (i, j, k) in the staged plane
     const int ny1 = v.ny + 1, nz1 = v.nz + 1;
     for (int i = t.i0; i < t.i1; ++i) {
-      stage(i);
+      store(i, f);
       __syncthreads();
+      if (i + 1 < t.i1) fetch(i + 1, f);          // in flight while plane i is computed
       if (j < v.ny && k < v.nz) {
         const int s0 = i & 1, sm = s0 ^ 1;          // planes i and i-1
         auto Y = [&](int s, int d, int dj, int dk) { return sy[s][d][q + dj * PK + dk]; };
@@ -377,30 +440,63 @@ __global__ void k_alpha2(const double* __restrict__ pa, int na, const double* __
 }
 
 // ------------------------------------------------------ CG on the grid
+// 16-byte vector of T (two doubles or four floats) for the streaming CG updates
+template <typename T> struct V16;
+template <> struct V16<double> { using type = double2; static constexpr int n = 2; };
+template <> struct V16<float> { using type = float4; static constexpr int n = 4; };
+template <typename T>
+union Pack {
+  typename V16<T>::type v;
+  T a[V16<T>::n];
+};
+
+// x += alpha p, r -= alpha w, block partials of r.prec.r; 16-byte loads and stores, each
+// thread kPerThread vectors per chunk (the tail of n is done element by element)
 template <typename T>
 __global__ void k_gcg_update(int64_t n, const double* __restrict__ scal, T* __restrict__ x, T* __restrict__ r,
                              const T* __restrict__ p, const T* __restrict__ w, const T* __restrict__ prec,
                              double* __restrict__ partials) {
   __shared__ double sh[kFinBlock / 64];
+  using VT = typename V16<T>::type;
+  constexpr int NV = V16<T>::n;
   const T alf = (T)scal[S_ALPHA];
   double acc = 0.0;
+  const int64_t nvec = n / NV;
   const int64_t chunk = (int64_t)blockDim.x * kPerThread;
-  for (int64_t base = blockIdx.x * chunk; base < n; base += (int64_t)gridDim.x * chunk) {
-    T xv[kPerThread], pv[kPerThread], rv[kPerThread], wv[kPerThread], mv[kPerThread];
+  VT* xv = reinterpret_cast<VT*>(x);
+  VT* rv = reinterpret_cast<VT*>(r);
+  const VT* pv = reinterpret_cast<const VT*>(p);
+  const VT* wv = reinterpret_cast<const VT*>(w);
+  const VT* mv = reinterpret_cast<const VT*>(prec);
+  for (int64_t base = blockIdx.x * chunk; base < nvec; base += (int64_t)gridDim.x * chunk) {
+    Pack<T> X[kPerThread], P[kPerThread], R[kPerThread], W[kPerThread], M[kPerThread];
 #pragma unroll
     for (int u = 0; u < kPerThread; ++u) {
       const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
-      if (i < n) { xv[u] = x[i]; pv[u] = p[i]; rv[u] = r[i]; wv[u] = w[i]; mv[u] = prec[i]; }
+      if (i < nvec) { X[u].v = xv[i]; P[u].v = pv[i]; R[u].v = rv[i]; W[u].v = wv[i]; M[u].v = mv[i]; }
     }
 #pragma unroll
     for (int u = 0; u < kPerThread; ++u) {
       const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
-      if (i < n) {
-        x[i] = xv[u] + alf * pv[u];
-        const T ri = rv[u] - alf * wv[u];
-        r[i] = ri;
-        acc += (double)(ri * mv[u]) * (double)ri;
+      if (i < nvec) {
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+          X[u].a[e] = X[u].a[e] + alf * P[u].a[e];
+          const T ri = R[u].a[e] - alf * W[u].a[e];
+          R[u].a[e] = ri;
+          acc += (double)(ri * M[u].a[e]) * (double)ri;
+        }
+        xv[i] = X[u].v;
+        rv[i] = R[u].v;
       }
+    }
+  }
+  if (blockIdx.x == 0) {
+    for (int64_t i = nvec * NV + threadIdx.x; i < n; i += blockDim.x) {
+      x[i] = x[i] + alf * p[i];
+      const T ri = r[i] - alf * w[i];
+      r[i] = ri;
+      acc += (double)(ri * prec[i]) * (double)ri;
     }
   }
   double s = block_sum(acc, sh);
@@ -417,12 +513,26 @@ __global__ void k_gcg_beta(const double* __restrict__ partials, int nb, double* 
     scal[S_BETA] = en / (eta != 0.0 ? eta : __builtin_inf());
   }
 }
+// p = r prec + beta p with 16-byte loads and stores
 template <typename T>
 __global__ void k_gcg_dir(int64_t n, const double* __restrict__ scal, const T* __restrict__ r,
                           const T* __restrict__ prec, T* __restrict__ p) {
+  using VT = typename V16<T>::type;
+  constexpr int NV = V16<T>::n;
   const T beta = (T)scal[S_BETA];
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = r[i] * prec[i] + beta * p[i];
+  const int64_t nvec = n / NV;
+  const VT* rv = reinterpret_cast<const VT*>(r);
+  const VT* mv = reinterpret_cast<const VT*>(prec);
+  VT* pv = reinterpret_cast<VT*>(p);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nvec; i += (int64_t)gridDim.x * blockDim.x) {
+    Pack<T> R, M, P;
+    R.v = rv[i]; M.v = mv[i]; P.v = pv[i];
+#pragma unroll
+    for (int e = 0; e < NV; ++e) P.a[e] = R.a[e] * M.a[e] + beta * P.a[e];
+    pv[i] = P.v;
+  }
+  if (blockIdx.x == 0)
+    for (int64_t i = nvec * NV + threadIdx.x; i < n; i += blockDim.x) p[i] = r[i] * prec[i] + beta * p[i];
 }
 __global__ void k_gcg_shift(double* __restrict__ scal) { scal[S_ETA] = scal[S_ETA_NEW]; }
 // residual replacement: eta_new <- the replaced residual's eta, beta from it
@@ -538,7 +648,7 @@ int gcg_iterate(const pyib_grid* g, const pyib_work* ws, const pyib_cg* cg, doub
   T* w = static_cast<T*>(cg->w);
   const T* prec = static_cast<const T*>(cg->prec);
   const int64_t n = g->nedge;
-  const int nb = red_grid_for(n);
+  const int nb = red_grid_for(n / V16<T>::n + 1);   // the update streams 16-byte vectors
   // the update's partials use a third row so they never alias the dot of A p
   double* upart = ws->partials + 2 * kRedGrid;
   for (int it = 0; it < n_iter; ++it) {
@@ -548,7 +658,7 @@ int gcg_iterate(const pyib_grid* g, const pyib_work* ws, const pyib_cg* cg, doub
     k_gcg_update<T><<<nb, kBlock, 0, st>>>(n, cg->scal, x, r, p, w, prec, upart);
     k_gcg_beta<<<1, kFinBlock, 0, st>>>(upart, nb, cg->scal);
     if (!(skip_last && it == n_iter - 1)) {
-      k_gcg_dir<T><<<grid_for(n), kBlock, 0, st>>>(n, cg->scal, r, prec, p);
+      k_gcg_dir<T><<<grid_for(n / V16<T>::n + 1), kBlock, 0, st>>>(n, cg->scal, r, prec, p);
       k_gcg_shift<<<1, 1, 0, st>>>(cg->scal);
     }
   }
@@ -602,10 +712,10 @@ int pyib_grid_cg_iterate(const pyib_grid* g, const pyib_work* ws, const pyib_cg*
 int pyib_grid_cg_direction(const pyib_grid* g, const pyib_cg* cg, void* stream) {
   auto st = S(stream);
   if (g->dtype == PYIB_F64)
-    k_gcg_dir<double><<<grid_for(g->nedge), kBlock, 0, st>>>(g->nedge, cg->scal, (const double*)cg->r,
+    k_gcg_dir<double><<<grid_for(g->nedge / 2 + 1), kBlock, 0, st>>>(g->nedge, cg->scal, (const double*)cg->r,
                                                              (const double*)cg->prec, (double*)cg->p);
   else
-    k_gcg_dir<float><<<grid_for(g->nedge), kBlock, 0, st>>>(g->nedge, cg->scal, (const float*)cg->r,
+    k_gcg_dir<float><<<grid_for(g->nedge / 4 + 1), kBlock, 0, st>>>(g->nedge, cg->scal, (const float*)cg->r,
                                                             (const float*)cg->prec, (float*)cg->p);
   k_gcg_shift<<<1, 1, 0, st>>>(cg->scal);
   return check_launch("pyib_grid_cg_direction");
