@@ -39,7 +39,17 @@ Levels levels_of(const pyib_grid* g) {
 }
 
 // ---------------------------------------------------------------- G1
-// composite rows in sliced ELL, one wavefront per slice of 64 rows (lane = row):
+// composite rows in sliced ELL, one wavefront per slice of 64 rows (lane = row).  The slice
+// index is wave-uniform (scalar loads of width/sptr); each lane first loads up to kSellBatch
+// (column, coefficient) pairs, then issues all their gathers, so a row costs about two
+// memory round trips per batch instead of two per entry.  Entries are summed in order.
+constexpr int kSellBatch = 8;
+
+__device__ __forceinline__ int64_t wave_slice() {
+  return __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+}
+
+// ---------------------------------------------------------------- G1
 // vc_d(row) = sum_j coef_d * p[col]
 template <typename T>
 __global__ void k_sell_g1(int64_t nslice, int64_t ncell, const int32_t* __restrict__ row,
@@ -47,20 +57,31 @@ __global__ void k_sell_g1(int64_t nslice, int64_t ncell, const int32_t* __restri
                           const int32_t* __restrict__ col, const T* __restrict__ coef, const T* __restrict__ p,
                           T* __restrict__ vc) {
   const int lane = threadIdx.x & 63;
-  const int64_t wpb = blockDim.x >> 6;
-  for (int64_t s = blockIdx.x * wpb + (threadIdx.x >> 6); s < nslice; s += (int64_t)gridDim.x * wpb) {
-    const int W = width[s];
-    const int64_t b = sptr[s] + lane;
-    T a0 = 0, a1 = 0, a2 = 0;
-#pragma unroll 4
-    for (int j = 0; j < W; ++j) {
-      const int64_t e = b + (int64_t)j * 64;
-      const T x = p[col[e]];
-      a0 += coef[3 * e] * x; a1 += coef[3 * e + 1] * x; a2 += coef[3 * e + 2] * x;
+  const int64_t s = wave_slice();
+  if (s >= nslice) return;
+  const int W = width[s];
+  const int64_t b = sptr[s] + lane;
+  const int r = row[s * 64 + lane];
+  T a0 = 0, a1 = 0, a2 = 0;
+  for (int j0 = 0; j0 < W; j0 += kSellBatch) {
+    int cc[kSellBatch];
+    T c0[kSellBatch], c1[kSellBatch], c2[kSellBatch], xv[kSellBatch];
+#pragma unroll
+    for (int u = 0; u < kSellBatch; ++u) {
+      if (j0 + u < W) {
+        const int64_t e = b + (int64_t)(j0 + u) * 64;
+        cc[u] = col[e];
+        c0[u] = coef[3 * e]; c1[u] = coef[3 * e + 1]; c2[u] = coef[3 * e + 2];
+      }
     }
-    const int r = row[s * 64 + lane];
-    if (r >= 0) { vc[r] = a0; vc[ncell + r] = a1; vc[2 * ncell + r] = a2; }
+#pragma unroll
+    for (int u = 0; u < kSellBatch; ++u)
+      if (j0 + u < W) xv[u] = p[cc[u]];
+#pragma unroll
+    for (int u = 0; u < kSellBatch; ++u)
+      if (j0 + u < W) { a0 += c0[u] * xv[u]; a1 += c1[u] * xv[u]; a2 += c2[u] * xv[u]; }
   }
+  if (r >= 0) { vc[r] = a0; vc[ncell + r] = a1; vc[2 * ncell + r] = a2; }
 }
 
 // ----------------------------------------------------------------- V
@@ -70,28 +91,39 @@ __global__ void k_sell_v(int64_t nslice, int64_t ncell, const int32_t* __restric
                          const int32_t* __restrict__ col, const T* __restrict__ coef, T rou, T kv,
                          const T* __restrict__ x, T* __restrict__ y) {
   const int lane = threadIdx.x & 63;
-  const int64_t wpb = blockDim.x >> 6;
-  for (int64_t s = blockIdx.x * wpb + (threadIdx.x >> 6); s < nslice; s += (int64_t)gridDim.x * wpb) {
-    const int W = width[s];
-    const int64_t b = sptr[s] + lane;
-    T a0 = 0, a1 = 0, a2 = 0;
-#pragma unroll 4
-    for (int j = 0; j < W; ++j) {
-      const int64_t e = b + (int64_t)j * 64;
-      const int64_t c = col[e];
-      const T cf = coef[e];
-      a0 += cf * x[c]; a1 += cf * x[ncell + c]; a2 += cf * x[2 * ncell + c];
+  const int64_t s = wave_slice();
+  if (s >= nslice) return;
+  const int W = width[s];
+  const int64_t b = sptr[s] + lane;
+  const int r = row[s * 64 + lane];
+  T a0 = 0, a1 = 0, a2 = 0;
+  for (int j0 = 0; j0 < W; j0 += kSellBatch) {
+    int cc[kSellBatch];
+    T cf[kSellBatch], x0[kSellBatch], x1[kSellBatch], x2[kSellBatch];
+#pragma unroll
+    for (int u = 0; u < kSellBatch; ++u) {
+      if (j0 + u < W) {
+        const int64_t e = b + (int64_t)(j0 + u) * 64;
+        cc[u] = col[e];
+        cf[u] = coef[e];
+      }
     }
-    const int r = row[s * 64 + lane];
-    if (r >= 0) {
-      y[r] = rou * x[r] - kv * a0;
-      y[ncell + r] = rou * x[ncell + r] - kv * a1;
-      y[2 * ncell + r] = rou * x[2 * ncell + r] - kv * a2;
-    }
+#pragma unroll
+    for (int u = 0; u < kSellBatch; ++u)
+      if (j0 + u < W) { x0[u] = x[cc[u]]; x1[u] = x[ncell + cc[u]]; x2[u] = x[2 * ncell + cc[u]]; }
+#pragma unroll
+    for (int u = 0; u < kSellBatch; ++u)
+      if (j0 + u < W) { a0 += cf[u] * x0[u]; a1 += cf[u] * x1[u]; a2 += cf[u] * x2[u]; }
+  }
+  if (r >= 0) {
+    y[r] = rou * x[r] - kv * a0;
+    y[ncell + r] = rou * x[ncell + r] - kv * a1;
+    y[2 * ncell + r] = rou * x[2 * ncell + r] - kv * a2;
   }
 }
 
 // ---------------------------------------------------------------- G3
+// w(row) = sum_j coef . y[col] (Dirichlet rows 0); block partials of p.w over these rows
 template <typename T>
 __global__ void k_sell_g3(int64_t nslice, int64_t ncell, const int32_t* __restrict__ row,
                           const int32_t* __restrict__ width, const int64_t* __restrict__ sptr,
@@ -100,21 +132,35 @@ __global__ void k_sell_g3(int64_t nslice, int64_t ncell, const int32_t* __restri
                           const T* __restrict__ p, double* __restrict__ partials) {
   __shared__ double sh[kFinBlock / 64];
   const int lane = threadIdx.x & 63;
-  const int64_t wpb = blockDim.x >> 6;
   double dot = 0.0;
-  for (int64_t s = blockIdx.x * wpb + (threadIdx.x >> 6); s < nslice; s += (int64_t)gridDim.x * wpb) {
+  for (int64_t s = wave_slice(); s < nslice; s += (int64_t)gridDim.x * (blockDim.x >> 6)) {
     const int W = width[s];
     const int64_t b = sptr[s] + lane;
-    T acc = 0;
-#pragma unroll 4
-    for (int j = 0; j < W; ++j) {
-      const int64_t e = b + (int64_t)j * 64;
-      const int64_t c = col[e];
-      T t = coef[3 * e] * y[c] + coef[3 * e + 1] * y[ncell + c];
-      t = t + coef[3 * e + 2] * y[2 * ncell + c];
-      acc += t;
-    }
     const int r = row[s * 64 + lane];
+    T acc = 0;
+    for (int j0 = 0; j0 < W; j0 += kSellBatch) {
+      int cc[kSellBatch];
+      T c0[kSellBatch], c1[kSellBatch], c2[kSellBatch], y0[kSellBatch], y1[kSellBatch], y2[kSellBatch];
+#pragma unroll
+      for (int u = 0; u < kSellBatch; ++u) {
+        if (j0 + u < W) {
+          const int64_t e = b + (int64_t)(j0 + u) * 64;
+          cc[u] = col[e];
+          c0[u] = coef[3 * e]; c1[u] = coef[3 * e + 1]; c2[u] = coef[3 * e + 2];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kSellBatch; ++u)
+        if (j0 + u < W) { y0[u] = y[cc[u]]; y1[u] = y[ncell + cc[u]]; y2[u] = y[2 * ncell + cc[u]]; }
+#pragma unroll
+      for (int u = 0; u < kSellBatch; ++u) {
+        if (j0 + u < W) {
+          T t = c0[u] * y0[u] + c1[u] * y1[u];
+          t = t + c2[u] * y2[u];
+          acc += t;
+        }
+      }
+    }
     if (r >= 0) {
       if (mask && mask[r]) acc = T(0);
       w[r] = acc;
@@ -513,26 +559,12 @@ __global__ void k_gcg_beta(const double* __restrict__ partials, int nb, double* 
     scal[S_BETA] = en / (eta != 0.0 ? eta : __builtin_inf());
   }
 }
-// p = r prec + beta p with 16-byte loads and stores
 template <typename T>
 __global__ void k_gcg_dir(int64_t n, const double* __restrict__ scal, const T* __restrict__ r,
                           const T* __restrict__ prec, T* __restrict__ p) {
-  using VT = typename V16<T>::type;
-  constexpr int NV = V16<T>::n;
   const T beta = (T)scal[S_BETA];
-  const int64_t nvec = n / NV;
-  const VT* rv = reinterpret_cast<const VT*>(r);
-  const VT* mv = reinterpret_cast<const VT*>(prec);
-  VT* pv = reinterpret_cast<VT*>(p);
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nvec; i += (int64_t)gridDim.x * blockDim.x) {
-    Pack<T> R, M, P;
-    R.v = rv[i]; M.v = mv[i]; P.v = pv[i];
-#pragma unroll
-    for (int e = 0; e < NV; ++e) P.a[e] = R.a[e] * M.a[e] + beta * P.a[e];
-    pv[i] = P.v;
-  }
-  if (blockIdx.x == 0)
-    for (int64_t i = nvec * NV + threadIdx.x; i < n; i += blockDim.x) p[i] = r[i] * prec[i] + beta * p[i];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = r[i] * prec[i] + beta * p[i];
 }
 __global__ void k_gcg_shift(double* __restrict__ scal) { scal[S_ETA] = scal[S_ETA_NEW]; }
 // residual replacement: eta_new <- the replaced residual's eta, beta from it
@@ -658,7 +690,7 @@ int gcg_iterate(const pyib_grid* g, const pyib_work* ws, const pyib_cg* cg, doub
     k_gcg_update<T><<<nb, kBlock, 0, st>>>(n, cg->scal, x, r, p, w, prec, upart);
     k_gcg_beta<<<1, kFinBlock, 0, st>>>(upart, nb, cg->scal);
     if (!(skip_last && it == n_iter - 1)) {
-      k_gcg_dir<T><<<grid_for(n / V16<T>::n + 1), kBlock, 0, st>>>(n, cg->scal, r, prec, p);
+      k_gcg_dir<T><<<grid_for(n), kBlock, 0, st>>>(n, cg->scal, r, prec, p);
       k_gcg_shift<<<1, 1, 0, st>>>(cg->scal);
     }
   }
@@ -712,10 +744,10 @@ int pyib_grid_cg_iterate(const pyib_grid* g, const pyib_work* ws, const pyib_cg*
 int pyib_grid_cg_direction(const pyib_grid* g, const pyib_cg* cg, void* stream) {
   auto st = S(stream);
   if (g->dtype == PYIB_F64)
-    k_gcg_dir<double><<<grid_for(g->nedge / 2 + 1), kBlock, 0, st>>>(g->nedge, cg->scal, (const double*)cg->r,
+    k_gcg_dir<double><<<grid_for(g->nedge), kBlock, 0, st>>>(g->nedge, cg->scal, (const double*)cg->r,
                                                              (const double*)cg->prec, (double*)cg->p);
   else
-    k_gcg_dir<float><<<grid_for(g->nedge / 4 + 1), kBlock, 0, st>>>(g->nedge, cg->scal, (const float*)cg->r,
+    k_gcg_dir<float><<<grid_for(g->nedge), kBlock, 0, st>>>(g->nedge, cg->scal, (const float*)cg->r,
                                                             (const float*)cg->prec, (float*)cg->p);
   k_gcg_shift<<<1, 1, 0, st>>>(cg->scal);
   return check_launch("pyib_grid_cg_direction");

@@ -217,9 +217,8 @@ def _verify_regular(lay, mesh, geo, tables, tol, demote_neighbours=True):
 
     * G1 (cell <- edges): closed-form vol_ci, exactly six faces, and on each face this
       cell's rf (rf1 as first owner, rf2 as second) = +-h/2 e_d with four links;
-    * V (cell <- cells): closed-form vol_ci and six internal same-level faces with the full
-      closed form (rf1, rf2, AoL = h, Af = h^2, four links), at least ``margin`` cells from
-      the box and the hole;
+    * V (cell <- cells): closed-form vol_ci and six internal same-level faces with AoL = h,
+      at least ``margin`` cells from the box and the hole;
     * G3 (edge <- cells): an edge off the box boundary whose four adjacent cells are real
       and whose four faces (exactly four rot links) have closed-form rf1 and rf2.
     """
@@ -250,7 +249,9 @@ def _verify_regular(lay, mesh, geo, tables, tol, demote_neighbours=True):
     f = tables.cf_face
     g1_ent = np.where(first, rf1_ok[f], rf2_ok[f]) & links4[f]
     g1_ok = ok_vol & (cnt == 6) & (np.bincount(own, weights=(~g1_ent).astype(np.float64), minlength=ncell) == 0)
-    v_ok = ok_vol & (cnt == 6) & (np.bincount(own, weights=(~ok_face[f]).astype(np.float64), minlength=ncell) == 0)
+    # V reads only vol_ci, AoL and the six same-level neighbours
+    v_face = same_h & _close(np.abs(geo.AoL), hf, tol, hf)
+    v_ok = ok_vol & (cnt == 6) & (np.bincount(own, weights=(~v_face[f]).astype(np.float64), minlength=ncell) == 0)
     real = np.zeros(lay.ncell, bool)
     real[lay.cell_map] = True
     reg_g1 = np.zeros(lay.ncell, bool)

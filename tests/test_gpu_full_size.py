@@ -1,0 +1,86 @@
+"""The full-size configurations on the GPU (opt-in: PYIB_HIP_FULL_SIZE=1; minutes each).
+
+The oracle cannot run these sizes in test time, so they are checked through
+size-independent properties: every Helmholtz solve converges to the reference's
+own stop test, the force balance of the diagnostics holds (it raises otherwise),
+the face flux stays discretely divergence-free, repeated runs are bitwise
+identical, mixed precision agrees with float64 on the force history, and the
+adjoint gradient agrees with central finite differences.
+"""
+import os
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(os.environ.get("PYIB_HIP_FULL_SIZE") != "1",
+                                 reason="full-size runs are opt-in (PYIB_HIP_FULL_SIZE=1)")]
+
+import pyib_hip  # noqa: E402,F401
+import pyib_hip.autodiff as AD  # noqa: E402
+import pyib_hip.case as HC  # noqa: E402
+from tests.helpers import full_size_config  # noqa: E402
+
+
+def _steps(cfg, n, mesh=None):
+    rt = HC.build_runtime(cfg, mesh=mesh)
+    s, s_old, vc = HC.initial_state(rt)
+    rows = []
+    for step in range(1, n + 1):
+        rt.driver.time = (step - 1) * rt.dt
+        s, vc, s_old = rt.driver.step(s, s_old, rt.dt, rt.dt, inner=rt.inner_iterations)
+        rt.driver.time = step * rt.dt
+        rt.load_correction.advance(vc, HC.sphere_center(rt.config, rt.driver.time), rt.dt, rt.driver.ib_force_E)
+        row = HC.diagnostics(rt, step, s, vc, 0.0)
+        rows.append(row)
+        print(f"[gpu] {cfg['name']} step {step}: CG {row['helmholtz_iterations']} (sum "
+              f"{rt.driver.helmholtz_iterations_sum_last_step}), Fx {row['body_force_x']:.12e}, "
+              f"mass residual {row['mass_residual_inf']:.3e}", flush=True)
+        assert rt.driver.helmholtz_converged_last_step
+    return rt, s, vc, rows
+
+
+def test_cfg2_fixed_sphere_re100_d16():
+    cfg = full_size_config("cfg2", steps=2)
+    rt, s, vc, rows = _steps(cfg, 2)
+    rt2, s2, _, rows2 = _steps(cfg, 2, mesh=rt.mesh)
+    assert torch.equal(s, s2)                                  # deterministic, bitwise
+    assert all(r["mass_residual_inf"] < 1e-8 for r in rows)
+
+
+def test_cfg4_d40_mixed_vs_float64():
+    c64 = full_size_config("cfg4", steps=1)
+    rt, s64, vc64, r64 = _steps(c64, 1)
+    mesh = rt.mesh
+    U64 = rt.core.curl(s64).cpu().numpy()
+    del rt
+    torch.cuda.empty_cache()
+    cmx = full_size_config("cfg4", steps=1, precision="mixed")
+    rtm, smx, vcmx, rmx = _steps(cmx, 1, mesh=mesh)
+    Umx = rtm.core.curl(smx).cpu().numpy()
+    f64 = np.array([[r[c] for c in ("body_force_x", "body_force_y", "body_force_z")] for r in r64])
+    fmx = np.array([[r[c] for c in ("body_force_x", "body_force_y", "body_force_z")] for r in rmx])
+    ferr = float(np.linalg.norm(fmx - f64) / np.linalg.norm(f64))
+    uerr = float(np.linalg.norm(Umx - U64) / np.linalg.norm(U64))
+    print(f"[gpu] D40 mixed vs float64: force rel err {ferr:.2e}, face-flux L2 rel err {uerr:.2e}")
+    assert ferr <= 1e-6
+
+
+def test_cfg5_adjoint_d16_against_finite_differences():
+    steps, every = 10, 5
+    cfg = full_size_config("cfg5", steps=steps)
+    rt = HC.build_runtime(cfg)
+    HC.initial_state(rt)
+    obj = AD.PeriodForceObjective(rt, steps, checkpoint_every=every, tol1=1e-13, tol2=1e-12, cg_max=200000)
+    J, gA, gX = obj.value_and_grad()
+    D = 2.0 * float(rt.config["ib"]["radius"])
+    eps = 1e-6 * D
+    fdA = AD.central_difference(obj, direction_A=1.0, eps=eps)
+    d = AD.direction(gX.shape[0], eps)
+    fdX = AD.central_difference(obj, direction_X=d / eps, eps=eps)
+    adX = float((gX * d).sum()) / eps
+    rA, rX = abs(gA - fdA) / abs(fdA), abs(adX - fdX) / abs(fdX)
+    print(f"[gpu] D16 adjoint, K={steps}: J={J:.12e}, dJ/dA {gA:.10e} vs fd {fdA:.10e} (rel {rA:.2e}); "
+          f"dJ/dX0.d {adX:.10e} vs fd {fdX:.10e} (rel {rX:.2e})", flush=True)
+    assert rA <= 1e-5 and rX <= 1e-5
