@@ -9,12 +9,17 @@ mixed tolerance (1e-10 / 1e-9, check cadence 20).  The run starts from rest at
 t = 0; W warm-up steps are run untimed, then K steps are timed with the device
 synchronised on both sides.
 
-Prints one JSON line.  `value` is the steps/s of the precision given by
---precision (default: mixed = float32 PCG inside float64 defect correction);
-the float64 path's steps/s is reported beside it when both are run.
+Prints one JSON line.  `value` is the steps/s of the mixed-precision path
+(float32 PCG inside float64 reliable updates) when it is run (the default runs
+both), else of the float64 path; both paths' steps/s, the mixed-vs-float64
+force error and the speedup are reported beside it.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|float64|both]
+  python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--precision both|mixed|float64]
                   [--no-cpu-baseline] [--cpu-seconds 20]
+
+`roofline.traffic` is the measured HBM traffic of one PCG iteration (rocprofv3 FETCH_SIZE
+and WRITE_SIZE, scaled by a plain copy kernel), read from profiles/pcg_iteration_traffic_d32.md
+when that summary is present.
 """
 from __future__ import annotations
 
@@ -150,6 +155,16 @@ def run_gpu(precision, steps, warmup, log, mesh, mixed_inner_tol=None):
     return out
 
 
+def measured_traffic(prec):
+    """HBM bytes of one PCG iteration from the committed rocprofv3 summary, or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "pcg_iteration_traffic_d32.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(rf"traffic_bytes_per_iteration {prec} = ([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
 def cpu_baseline(mesh, geo, cfg, cg_iters_per_step, seconds, log):
     """Oracle (NumPy, one core for bincount/ufuncs) on a bounded sample of the same workload."""
     import numpy as np
@@ -240,9 +255,9 @@ def kernels_only(n_iter, precision, log):
 def main(argv=None):
     p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--precision", choices=("mixed", "float64", "both"), default="float64")
+    p.add_argument("--precision", choices=("mixed", "float64", "both"), default="both")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=20.0)
     p.add_argument("--kernels-only", type=int, default=0, help="profiling: N PCG iterations + a copy kernel")
@@ -277,7 +292,7 @@ def main(argv=None):
                                "A=0.1D, dt=T/400, 4 inner CN iterations, PCG tol 1e-10/1e-9",
                    "precision": head, "cg_iters_per_step": r["cg_iters_per_step"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(inner_prec),
                      "kernel": f"one {inner_prec} PCG iteration (A.p + dot, update + dot, direction)",
                      "ms_per_iteration": cg["ms_per_iter"],
                      "algorithmic_bytes_per_iteration": bytes_iter},

@@ -178,8 +178,8 @@ __global__ void k_sell_g3(int64_t nslice, int64_t ncell, const int32_t* __restri
 // CI planes in i (grid: x = tiles of the largest level, y = i-chunks, z = level).  The
 // planes it needs (with their halo in j and k) are staged once in LDS per step, so every
 // operand is read from memory about once instead of once per stencil tap.  Values outside
-// the level box are staged as 0: only rows away from the box edge are regular, and the
-// composite pass that follows overwrites the others.
+// the level box are staged as 0 (only rows away from the box edge are regular).  Only the
+// regular rows are written; the composite pass writes the others.
 constexpr int TJ = 8, TK = 32, CI = 16;        // 256 threads: tj = tid / TK, tk = tid % TK
 
 // grid of the tiled passes; *ci = planes per block (CI, doubled until the block count fits
@@ -228,8 +228,8 @@ struct Prefetch {
 // ---- G1: vc = interp(curl p) on the cells of plane i (reads x-edges of plane i, y/z-edges
 // of planes i and i+1)
 template <typename T>
-__global__ __launch_bounds__(256) void k_sg1_t(int64_t ncell, Levels L, int ci, const T* __restrict__ p,
-                                               T* __restrict__ vc) {
+__global__ __launch_bounds__(256) void k_sg1_t(int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
+                                               const T* __restrict__ p, T* __restrict__ vc) {
   const Lev& v = L.l[blockIdx.z];
   const Tile t = tile_of(v.ny, v.nz, v.nx, ci);
   if (!t.live) return;
@@ -304,9 +304,11 @@ __global__ __launch_bounds__(256) void k_sg1_t(int64_t ncell, Levels L, int ci, 
       const T uz0 = (PY(b, 0) - PY(a, 0)) - (PX(1, 0) - PX(0, 0));
       const T uz1 = (PY(b, 1) - PY(a, 1)) - (PX(1, 1) - PX(0, 1));
       const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
-      vc[c] = (ux0 + ux1) * cf;
-      vc[ncell + c] = (uy0 + uy1) * cf;
-      vc[2 * ncell + c] = (uz0 + uz1) * cf;
+      if (reg[c]) {
+        vc[c] = (ux0 + ux1) * cf;
+        vc[ncell + c] = (uy0 + uy1) * cf;
+        vc[2 * ncell + c] = (uz0 + uz1) * cf;
+      }
     }
     __syncthreads();
   }
@@ -314,8 +316,8 @@ __global__ __launch_bounds__(256) void k_sg1_t(int64_t ncell, Levels L, int ci, 
 
 // ---- V: y = rou x - (hdt/h^3)(a - b) on the cells of plane i (planes i-1, i, i+1 of x)
 template <typename T>
-__global__ __launch_bounds__(256) void k_sv_t(int64_t ncell, Levels L, int ci, T rou, T visc, T hdt,
-                                              const T* __restrict__ x, T* __restrict__ y) {
+__global__ __launch_bounds__(256) void k_sv_t(int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
+                                              T rou, T visc, T hdt, const T* __restrict__ x, T* __restrict__ y) {
   const Lev& v = L.l[blockIdx.z];
   const Tile t = tile_of(v.ny, v.nz, v.nx, ci);
   if (!t.live) return;
@@ -357,9 +359,9 @@ __global__ __launch_bounds__(256) void k_sv_t(int64_t ncell, Levels L, int ci, T
     store(i + 1, f);
     __syncthreads();
     if (i + 1 < t.i1) fetch(i + 2, f);          // in flight while plane i is computed
-    if (j < v.ny && k < v.nz) {
+    const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
+    if (j < v.ny && k < v.nz && reg[c]) {
       const int sm = (i + 2) % 3, s0 = i % 3, sp = (i + 1) % 3;
-      const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const T xc = sx[s0][d][q];
@@ -374,9 +376,8 @@ __global__ __launch_bounds__(256) void k_sv_t(int64_t ncell, Levels L, int ci, T
   }
 }
 
-// ---- G3: w on the x-, y- and z-edges of plane i (planes i-1 and i of y); irregular rows
-// and empty slots get 0 (the composite pass writes the irregular rows), rows on the box's
-// last plane/row are never regular and are left to the composite pass
+// ---- G3: w on the regular x-, y- and z-edges of plane i (planes i-1 and i of y); the
+// composite pass writes the irregular rows, empty slots are never written
 template <typename T>
 __global__ __launch_bounds__(256) void k_sg3_t(int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
                                                const T* __restrict__ y, T* __restrict__ w, const T* __restrict__ p,
@@ -431,9 +432,11 @@ __global__ __launch_bounds__(256) void k_sg3_t(int64_t ncell, Levels L, int ci, 
           const T fy1 = hh * (Y(s0, 1, -1, -1) + Y(s0, 1, 0, -1));
           const T fz0 = hh * (Y(s0, 2, 0, -1) + Y(s0, 2, 0, 0));
           const T fz1 = hh * (Y(s0, 2, -1, -1) + Y(s0, 2, -1, 0));
-          const T val = reg[e] ? (fy1 - fy0) + (fz0 - fz1) : T(0);
-          w[e] = val;
-          if (p) dot += (double)p[e] * (double)val;
+          if (reg[e]) {
+            const T val = (fy1 - fy0) + (fz0 - fz1);
+            w[e] = val;
+            if (p) dot += (double)p[e] * (double)val;
+          }
         }
         {   // y-edge (i, j, k): F_x(i,j,k) - F_x(i,j,k-1) - F_z(i,j,k) + F_z(i-1,j,k)
           const int64_t e = v.e1 + ((int64_t)i * v.ny + j) * nz1 + k;
@@ -441,9 +444,11 @@ __global__ __launch_bounds__(256) void k_sg3_t(int64_t ncell, Levels L, int ci, 
           const T fx1 = hh * (Y(sm, 0, 0, -1) + Y(s0, 0, 0, -1));
           const T fz0 = hh * (Y(s0, 2, 0, -1) + Y(s0, 2, 0, 0));
           const T fz1 = hh * (Y(sm, 2, 0, -1) + Y(sm, 2, 0, 0));
-          const T val = reg[e] ? (fx0 - fx1) + (fz1 - fz0) : T(0);
-          w[e] = val;
-          if (p) dot += (double)p[e] * (double)val;
+          if (reg[e]) {
+            const T val = (fx0 - fx1) + (fz1 - fz0);
+            w[e] = val;
+            if (p) dot += (double)p[e] * (double)val;
+          }
         }
         {   // z-edge (i, j, k): -F_x(i,j,k) + F_x(i,j-1,k) + F_y(i,j,k) - F_y(i-1,j,k)
           const int64_t e = v.e2 + ((int64_t)i * ny1 + j) * v.nz + k;
@@ -451,9 +456,11 @@ __global__ __launch_bounds__(256) void k_sg3_t(int64_t ncell, Levels L, int ci, 
           const T fx1 = hh * (Y(sm, 0, -1, 0) + Y(s0, 0, -1, 0));
           const T fy0 = hh * (Y(s0, 1, -1, 0) + Y(s0, 1, 0, 0));
           const T fy1 = hh * (Y(sm, 1, -1, 0) + Y(sm, 1, 0, 0));
-          const T val = reg[e] ? (fx1 - fx0) + (fy0 - fy1) : T(0);
-          w[e] = val;
-          if (p) dot += (double)p[e] * (double)val;
+          if (reg[e]) {
+            const T val = (fx1 - fx0) + (fy0 - fy1);
+            w[e] = val;
+            if (p) dot += (double)p[e] * (double)val;
+          }
         }
       }
       __syncthreads();
@@ -642,6 +649,9 @@ __global__ void k_permute(int64_t n, const int64_t* __restrict__ idx, const int8
 // ------------------------------------------------------- launchers
 inline int slice_grid(int64_t nslice) { return grid_for(nslice * 64); }
 inline int slice_red_grid(int64_t nslice) { return red_grid_for(nslice * 64); }
+// w = A p on the grid; the p.w partials go to ws->partials [0, nb1) and [kRedGrid, kRedGrid+nb2).
+// Each pass: the tiled stencil writes the regular rows, the composite kernel the others
+// (disjoint rows).  Running the two on separate streams measured no faster than in order.
 template <typename T>
 int gflow(const pyib_grid* g, const pyib_work* ws, double rou, double visc, double hdt, const T* p, T* w,
           bool want_dot, int* nb1, int* nb2, hipStream_t st) {
@@ -650,11 +660,11 @@ int gflow(const pyib_grid* g, const pyib_work* ws, double rou, double visc, doub
   T* cb = static_cast<T*>(ws->cell_b);
   int ci = CI;
   const dim3 gt = tile_grid(L, &ci);
-  k_sg1_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, p, ca);
+  k_sg1_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, g->cell_reg_g1, p, ca);
   if (g->g1_nslice > 0)
     k_sell_g1<T><<<slice_grid(g->g1_nslice), kBlock, 0, st>>>(g->g1_nslice, g->ncell, g->g1_row, g->g1_width,
                                                              g->g1_sptr, g->g1_col, (const T*)g->g1_coef, p, ca);
-  k_sv_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, (T)rou, (T)visc, (T)hdt, ca, cb);
+  k_sv_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, g->cell_reg, (T)rou, (T)visc, (T)hdt, ca, cb);
   if (g->v_nslice > 0)
     k_sell_v<T><<<slice_grid(g->v_nslice), kBlock, 0, st>>>(g->v_nslice, g->ncell, g->v_row, g->v_width, g->v_sptr,
                                                            g->v_col, (const T*)g->v_coef, (T)rou, (T)(hdt * visc),

@@ -28,6 +28,7 @@ class HipGrid:
         self.ncell, self.nedge = lay.ncell, lay.nedge
         t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)
         self._cell_reg = t(lay.reg_cell, torch.uint8)
+        self._cell_reg_g1 = t(lay.reg_g1, torch.uint8)
         self._edge_reg = t(lay.reg_edge, torch.uint8)
         self._dir = t(np.ones(lay.nedge, np.uint8), torch.uint8)
         # reference -> internal gather: idx over internal slots; internal -> reference: idx over ref edges
@@ -75,7 +76,8 @@ class HipGrid:
             L.cell_off = lv.cell_off
             L.e_off[0], L.e_off[1], L.e_off[2] = lv.e_off
             L.h = lv.h
-        g.cell_reg, g.edge_reg, g.dir = ptr(self._cell_reg), ptr(self._edge_reg), ptr(self._dir)
+        g.cell_reg, g.cell_reg_g1 = ptr(self._cell_reg), ptr(self._cell_reg_g1)
+        g.edge_reg, g.dir = ptr(self._edge_reg), ptr(self._dir)
         c, k = self._comp, self._coef[prec]
         for pre, name in (("g1", "g1"), ("v", "vis"), ("g3", "g3")):
             cc = c[name]

@@ -102,8 +102,10 @@ typedef struct pyib_grid {
   int32_t nlevels;
   int64_t ncell, nedge;     /* internal (padded) sizes */
   pyib_level lev[PYIB_MAX_LEVELS];
-  const uint8_t *cell_reg;  /* ncell: row computed by the stencil */
-  const uint8_t *edge_reg;  /* nedge: row computed by the stencil */
+  const uint8_t *cell_reg;  /* ncell: V row computed by the stencil */
+  const uint8_t *cell_reg_g1; /* ncell: G1 row computed by the stencil */
+  const uint8_t *edge_reg;  /* nedge: G3 row computed by the stencil (w must be zero-initialised:
+                             * empty slots are never written) */
   const uint8_t *dir;       /* nedge: Dirichlet edge or empty slot (residual forced to 0) */
   /* composite rows of the other (interface/boundary) rows in sliced ELL: slices of 64
    * rows (one wavefront), entry j of lane l of slice s at sptr[s] + 64*j + l, width[s]
