@@ -203,13 +203,25 @@ struct Tile {
   bool live;
 };
 // tile of this block for a level whose (j, k) extent is (ny_t, nz_t) and i extent ni
-__device__ __forceinline__ Tile tile_of(int ny_t, int nz_t, int ni, int ci) {
+// block ids remapped so that each XCD (workgroups go round-robin over the 8 XCDs) gets one
+// contiguous run of tiles: neighbouring tiles share that XCD's L2 for their halo lines
+__device__ __forceinline__ void xcd_ids(int& bx, int& by, int& bz) {
+  const int64_t gx = gridDim.x, gy = gridDim.y, nb = gx * gy * gridDim.z;
+  const int64_t b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const int64_t q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+  const int64_t lb = x * q + (x < r ? x : r) + i;
+  bx = (int)(lb % gx);
+  by = (int)((lb / gx) % gy);
+  bz = (int)(lb / (gx * gy));
+}
+
+__device__ __forceinline__ Tile tile_of(int ny_t, int nz_t, int ni, int ci, int bx, int by) {
   Tile t;
   const int ntk = (nz_t + TK - 1) / TK, ntj = (ny_t + TJ - 1) / TJ;
-  const int b = blockIdx.x;
+  const int b = bx;
   t.j0 = (b / ntk) * TJ;
   t.k0 = (b % ntk) * TK;
-  t.i0 = blockIdx.y * ci;
+  t.i0 = by * ci;
   t.i1 = min(t.i0 + ci, ni);
   t.live = b < ntj * ntk && t.i0 < ni;
   t.tj = threadIdx.x / TK;
@@ -230,8 +242,10 @@ struct Prefetch {
 template <typename T>
 __global__ __launch_bounds__(256) void k_sg1_t(int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
                                                const T* __restrict__ p, T* __restrict__ vc) {
-  const Lev& v = L.l[blockIdx.z];
-  const Tile t = tile_of(v.ny, v.nz, v.nx, ci);
+  int bx, by, bz;
+  xcd_ids(bx, by, bz);
+  const Lev& v = L.l[bz];
+  const Tile t = tile_of(v.ny, v.nz, v.nx, ci, bx, by);
   if (!t.live) return;
   constexpr int XJ = TJ + 1, XK = TK + 1;       // x-edges: rows j..j+1, k..k+1
   constexpr int YJ = TJ, YK = TK + 1;           // y-edges: k..k+1
@@ -318,8 +332,10 @@ __global__ __launch_bounds__(256) void k_sg1_t(int64_t ncell, Levels L, int ci, 
 template <typename T>
 __global__ __launch_bounds__(256) void k_sv_t(int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
                                               T rou, T visc, T hdt, const T* __restrict__ x, T* __restrict__ y) {
-  const Lev& v = L.l[blockIdx.z];
-  const Tile t = tile_of(v.ny, v.nz, v.nx, ci);
+  int bx, by, bz;
+  xcd_ids(bx, by, bz);
+  const Lev& v = L.l[bz];
+  const Tile t = tile_of(v.ny, v.nz, v.nx, ci, bx, by);
   if (!t.live) return;
   constexpr int PJ = TJ + 2, PK = TK + 2, PS = PJ * PK, R = 2;
   __shared__ T sx[3][3][PS];                    // [plane slot][component][(j, k) with halo]
@@ -383,8 +399,10 @@ __global__ __launch_bounds__(256) void k_sg3_t(int64_t ncell, Levels L, int ci, 
                                                const T* __restrict__ y, T* __restrict__ w, const T* __restrict__ p,
                                                double* __restrict__ partials) {
   __shared__ double sh[kFinBlock / 64];
-  const Lev& v = L.l[blockIdx.z];
-  const Tile t = tile_of(v.ny, v.nz, v.nx, ci);
+  int bx, by, bz;
+  xcd_ids(bx, by, bz);
+  const Lev& v = L.l[bz];
+  const Tile t = tile_of(v.ny, v.nz, v.nx, ci, bx, by);
   double dot = 0.0;
   if (t.live) {
     constexpr int PJ = TJ + 1, PK = TK + 1, PS = PJ * PK, R = 2;   // low-side halo (j-1, k-1)
