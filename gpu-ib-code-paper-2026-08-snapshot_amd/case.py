@@ -479,7 +479,7 @@ def _setup_objects(config, mesh):
         params["fingerprint"] = hashlib.sha256(np.ascontiguousarray(mesh.nodes).tobytes()).hexdigest()
     geo = setup_cache.cached("geometry", params, lambda: Geometry(mesh), Geometry)
     tables = setup_cache.cached("tables", params, lambda: build_tables(geo), Tables)
-    lay = setup_cache.cached("layout", dict(params, layout_rules="per-pass-regularity-2"),
+    lay = setup_cache.cached("layout", dict(params, layout_rules="per-pass-regularity-3"),
                              lambda: build_layout(mesh, geo, tables), StructuredLayout,
                              {"Composite": Composite, "Level": Level})
     return geo, tables, lay

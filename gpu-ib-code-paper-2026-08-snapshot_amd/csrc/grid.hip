@@ -3,11 +3,12 @@
 // The CG of Driver.helm_solve (ib_driver_np.py:311-410) runs on the internal
 // layout: per level, dense x/y/z edge blocks and a cell block, +axis oriented.
 // A p = rot(integr(V(interp(curl p)))) (ib_core_np.py:213-220) is applied in
-// three passes.  Each pass first runs the constant-coefficient stencil of each
-// level over its whole box (2.5D tiles staged in LDS, the fastest index k on
-// consecutive lanes), then the composite rows (level interfaces, outer boundary,
-// sliced ELL) overwrite the rows the stencil does not describe.  Cell vectors
-// are stored component-major (SoA).
+// three passes.  In each pass the constant-coefficient stencil of each level
+// (2.5D tiles staged in LDS, the fastest index k on consecutive lanes) writes
+// the regular rows, and the composite rows (level interfaces, outer boundary,
+// sliced ELL) write the rows the stencil does not describe.  Cell vectors
+// are stored interleaved (x, y, z of a cell adjacent): the composite rows gather
+// whole cells, one cache line per cell instead of one per component.
 //
 //   G1  vc_d(c)  = (U_d(lo) + U_d(hi)) / (2h^2)       U = discrete curl of p
 //   V   y(c)     = rou x - (hdt/h^3)(a - b),  a = sum_up (h visc)(x_n - x), b = sum_dn (h visc)(x - x_n)
@@ -81,7 +82,7 @@ __global__ void k_sell_g1(int64_t nslice, int64_t ncell, const int32_t* __restri
     for (int u = 0; u < kSellBatch; ++u)
       if (j0 + u < W) { a0 += c0[u] * xv[u]; a1 += c1[u] * xv[u]; a2 += c2[u] * xv[u]; }
   }
-  if (r >= 0) { vc[r] = a0; vc[ncell + r] = a1; vc[2 * ncell + r] = a2; }
+  if (r >= 0) { vc[3 * (int64_t)r] = a0; vc[3 * (int64_t)r + 1] = a1; vc[3 * (int64_t)r + 2] = a2; }
 }
 
 // ----------------------------------------------------------------- V
@@ -110,15 +111,19 @@ __global__ void k_sell_v(int64_t nslice, int64_t ncell, const int32_t* __restric
     }
 #pragma unroll
     for (int u = 0; u < kSellBatch; ++u)
-      if (j0 + u < W) { x0[u] = x[cc[u]]; x1[u] = x[ncell + cc[u]]; x2[u] = x[2 * ncell + cc[u]]; }
+      if (j0 + u < W) {
+        const int64_t o = 3 * (int64_t)cc[u];
+        x0[u] = x[o]; x1[u] = x[o + 1]; x2[u] = x[o + 2];
+      }
 #pragma unroll
     for (int u = 0; u < kSellBatch; ++u)
       if (j0 + u < W) { a0 += cf[u] * x0[u]; a1 += cf[u] * x1[u]; a2 += cf[u] * x2[u]; }
   }
   if (r >= 0) {
-    y[r] = rou * x[r] - kv * a0;
-    y[ncell + r] = rou * x[ncell + r] - kv * a1;
-    y[2 * ncell + r] = rou * x[2 * ncell + r] - kv * a2;
+    const int64_t o = 3 * (int64_t)r;
+    y[o] = rou * x[o] - kv * a0;
+    y[o + 1] = rou * x[o + 1] - kv * a1;
+    y[o + 2] = rou * x[o + 2] - kv * a2;
   }
 }
 
@@ -151,7 +156,10 @@ __global__ void k_sell_g3(int64_t nslice, int64_t ncell, const int32_t* __restri
       }
 #pragma unroll
       for (int u = 0; u < kSellBatch; ++u)
-        if (j0 + u < W) { y0[u] = y[cc[u]]; y1[u] = y[ncell + cc[u]]; y2[u] = y[2 * ncell + cc[u]]; }
+        if (j0 + u < W) {
+          const int64_t o = 3 * (int64_t)cc[u];
+          y0[u] = y[o]; y1[u] = y[o + 1]; y2[u] = y[o + 2];
+        }
 #pragma unroll
       for (int u = 0; u < kSellBatch; ++u) {
         if (j0 + u < W) {
@@ -319,9 +327,9 @@ __global__ __launch_bounds__(256) void k_sg1_t(int64_t ncell, Levels L, int ci, 
       const T uz1 = (PY(b, 1) - PY(a, 1)) - (PX(1, 1) - PX(0, 1));
       const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
       if (reg[c]) {
-        vc[c] = (ux0 + ux1) * cf;
-        vc[ncell + c] = (uy0 + uy1) * cf;
-        vc[2 * ncell + c] = (uz0 + uz1) * cf;
+        vc[3 * c] = (ux0 + ux1) * cf;
+        vc[3 * c + 1] = (uy0 + uy1) * cf;
+        vc[3 * c + 2] = (uz0 + uz1) * cf;
       }
     }
     __syncthreads();
@@ -347,7 +355,7 @@ __global__ __launch_bounds__(256) void k_sv_t(int64_t ncell, Levels L, int ci, c
       const bool in = q < PS && i >= 0 && i < v.nx && j >= 0 && j < v.ny && k >= 0 && k < v.nz;
       const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) f.v[3 * r + d] = in ? x[d * ncell + c] : T(0);
+      for (int d = 0; d < 3; ++d) f.v[3 * r + d] = in ? x[3 * c + d] : T(0);
     }
   };
   auto store = [&](int i, const Prefetch<T, 3 * R>& f) {
@@ -385,7 +393,7 @@ __global__ __launch_bounds__(256) void k_sv_t(int64_t ncell, Levels L, int ci, c
         a = a + kA * (sx[s0][d][q + 1] - xc);
         T b = kA * (xc - sx[sm][d][q]) + kA * (xc - sx[s0][d][q - PK]);
         b = b + kA * (xc - sx[s0][d][q - 1]);
-        y[d * ncell + c] = rou * xc - hv * (a - b);
+        y[3 * c + d] = rou * xc - hv * (a - b);
       }
     }
     __syncthreads();
@@ -415,7 +423,7 @@ __global__ __launch_bounds__(256) void k_sg3_t(int64_t ncell, Levels L, int ci, 
         const bool in = q < PS && i >= 0 && i < v.nx && j >= 0 && j < v.ny && k >= 0 && k < v.nz;
         const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) f.v[3 * r + d] = in ? y[d * ncell + c] : T(0);
+        for (int d = 0; d < 3; ++d) f.v[3 * r + d] = in ? y[3 * c + d] : T(0);
       }
     };
     auto store = [&](int i, const Prefetch<T, 3 * R>& f) {

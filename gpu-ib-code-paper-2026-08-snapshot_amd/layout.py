@@ -396,13 +396,15 @@ class SellComposite:
     coef: np.ndarray            # float64 (entries,) or (entries, 3), 0 on padding
 
 
-def to_sell(comp, C=SLICE):
-    """CSR composite -> sliced ELL.  Rows are ordered by decreasing length (stable, so rows
-    of one length keep their spatial order) to keep the padding small; each lane keeps its
-    row's entries in CSR order, so the per-row sums are unchanged."""
+def to_sell(comp, C=SLICE, sigma=8 * SLICE):
+    """CSR composite -> sliced ELL.  Inside windows of ``sigma`` consecutive rows (rows come
+    in internal, i.e. spatial, order) the rows are ordered by decreasing length, which keeps
+    the padding small while neighbouring lanes still gather from neighbouring cells; each
+    lane keeps its row's entries in CSR order, so the per-row sums are unchanged."""
     n = int(comp.rows.shape[0])
     width = np.diff(comp.ptr).astype(np.int64)
-    order = np.argsort(-width, kind="stable")
+    window = np.arange(n, dtype=np.int64) // max(int(sigma), 1)
+    order = np.lexsort((-width, window))
     nslice = (n + C - 1) // C
     w_sorted = np.zeros(nslice * C, np.int64)
     w_sorted[:n] = width[order]

@@ -185,3 +185,21 @@ def test_reference_d32_run_history():
             worst[col] = max(worst.get(col, 0.0), abs(a - b) / max(abs(b), 1e-300))
     print("[gpu] worst relative differences vs the reference's CUDA run:", worst)
     assert worst["body_force_x"] < 1e-8 and worst["force_fluid_x"] < 1e-8
+
+
+def test_mixed_precision_force_harmonics_over_a_period():
+    """Mixed vs float64 over one full period (D8 nested box, dt = T/100): the harmonic vector
+    (a cos + b sin + c fit of body_force_x over the period, the reference's validation fit
+    tools/check_pressure_validation.py:32-38) agrees to 1e-6 relative."""
+    from oracle.pyib_ref.case import harmonic_fit
+    f = 0.5729577951308232
+    steps = 100
+    over = {"time": {"dt": (1.0 / f) / steps, "steps": steps, "inner_iterations": 4}}
+    f64, _, _ = _forces(small_case("oscillating", steps=steps, **over), steps)
+    fmx, _, _ = _forces(small_case("oscillating", precision="mixed", steps=steps, **over), steps)
+    t = (1.0 / f) / steps * np.arange(1, steps + 1)
+    h64 = harmonic_fit(t, f64[:, 0], 2.0 * np.pi * f, 0.0)
+    hmx = harmonic_fit(t, fmx[:, 0], 2.0 * np.pi * f, 0.0)
+    err = float(np.linalg.norm(hmx - h64) / np.linalg.norm(h64))
+    print(f"[gpu] one period, mixed vs float64: harmonic (a, b, c) = {h64} vs {hmx}, rel err {err:.2e}")
+    assert err <= 1e-6
