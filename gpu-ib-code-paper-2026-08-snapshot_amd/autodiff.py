@@ -418,7 +418,8 @@ def step(K, A, X0, s, s_old, fL, n):
 class PeriodForceObjective:
     """J(A, X0) = (1/K) sum_{n=1..K} body_force_x(n) from rest, and its gradient."""
 
-    def __init__(self, rt, steps, checkpoint_every=20, tol1=1e-10, tol2=1e-9, cg_max=None):
+    def __init__(self, rt, steps, checkpoint_every=20, tol1=1e-10, tol2=1e-9, cg_max=None, log=None):
+        self.log = log                       # optional callable(str): one line per segment
         self.K = StepContext(rt, tol1=tol1, tol2=tol2, cg_max=cg_max)
         self.steps = int(steps)
         self.every = max(1, int(checkpoint_every))
@@ -456,6 +457,8 @@ class PeriodForceObjective:
         X0 = self.X00 if X0 is None else X0
         with torch.no_grad():
             acc, _ = self._segment(A, X0, self._initial(A, X0), 0, self.steps)
+        if self.log:
+            self.log(f"J evaluated: CG {self.K.cg_iterations}")
         return float(acc.item()) / self.steps
 
     def value_and_grad(self, A=None, X0=None):
@@ -474,6 +477,8 @@ class PeriodForceObjective:
                 states.append(tuple(x.clone() for x in state))
                 acc, state = self._segment(A_c, X0_val, state, a, b)
                 total += float(acc.item())
+                if self.log:
+                    self.log(f"forward steps {a + 1}-{b}: CG {K.cg_iterations}")
         gA = torch.zeros((), dtype=torch.float64, device=dev)
         gX = torch.zeros_like(X0_val)
         g_state = None
@@ -495,6 +500,9 @@ class PeriodForceObjective:
             gA = gA + (res[0] if res[0] is not None else 0.0)
             gX = gX + (res[1] if res[1] is not None else 0.0)
             g_state = res[2:]
+            if self.log:
+                self.log(f"backward steps {b}-{a + 1}: adjoint CG {K.adjoint_cg_iterations}, "
+                         f"projection CG {K.projection_iterations}")
         return total / self.steps, float(gA.item()), gX
 
 

@@ -456,20 +456,21 @@ def _mesh_params(config):
 _SETUP_MEMO = {}
 
 
-def setup_objects(config, mesh):
+def setup_objects(config, mesh, log=None):
     """Geometry, gather tables and structured layout of a mesh (cached on disk when
-    PYIB_HIP_SETUP_CACHE is set, and in this process for the mesh object last seen)."""
+    PYIB_HIP_SETUP_CACHE is set, and in this process for the mesh object last seen).
+    ``log`` (a callable taking one string) is told when each stage ends."""
     key = (id(mesh), json.dumps(_mesh_params(config), sort_keys=True))
     hit = _SETUP_MEMO.get(key)
     if hit is not None and hit[0] is mesh:
         return hit[1]
-    out = _setup_objects(config, mesh)
+    out = _setup_objects(config, mesh, log or (lambda m: None))
     _SETUP_MEMO.clear()
     _SETUP_MEMO[key] = (mesh, out)
     return out
 
 
-def _setup_objects(config, mesh):
+def _setup_objects(config, mesh, log):
     from . import setup_cache
     from .geometry import Geometry
     from .layout import Composite, Level, StructuredLayout, build_layout
@@ -477,11 +478,15 @@ def _setup_objects(config, mesh):
     params = _mesh_params(config)
     if config["mesh"]["kind"] == "uniform_box":
         params["fingerprint"] = hashlib.sha256(np.ascontiguousarray(mesh.nodes).tobytes()).hexdigest()
+    t0 = time.perf_counter()
     geo = setup_cache.cached("geometry", params, lambda: Geometry(mesh), Geometry)
+    log(f"geometry ready ({time.perf_counter() - t0:.1f} s)")
     tables = setup_cache.cached("tables", params, lambda: build_tables(geo), Tables)
+    log(f"gather tables ready ({time.perf_counter() - t0:.1f} s)")
     lay = setup_cache.cached("layout", dict(params, layout_rules="per-pass-regularity-3"),
                              lambda: build_layout(mesh, geo, tables), StructuredLayout,
                              {"Composite": Composite, "Level": Level})
+    log(f"structured layout ready ({time.perf_counter() - t0:.1f} s)")
     return geo, tables, lay
 
 
@@ -513,8 +518,9 @@ class CaseRuntime:
     source_hashes: dict
 
 
-def build_runtime(config, mesh=None, device="cuda"):
-    """ib_case.py:633-809 on the HIP path."""
+def build_runtime(config, mesh=None, device="cuda", log=None):
+    """ib_case.py:633-809 on the HIP path.  ``log`` (a callable taking one string) is told
+    when each set-up stage ends; a 40-cells-per-diameter mesh takes minutes to set up."""
     from .core import HipCore
     from .driver import HipDriver
     from .immersed import HipImmersedBoundary
@@ -525,17 +531,20 @@ def build_runtime(config, mesh=None, device="cuda"):
     ibc = config["ib"]
     radius = float(ibc["radius"])
     centre = np.asarray(ibc["center"], dtype=float)
+    say = log or (lambda m: None)
     if mesh is None:
         mesh, dx, summary = build_mesh(config)
+        say(f"mesh ready ({summary.get('total_cells', summary.get('cells'))} cells)")
     else:
         dx = float(np.asarray(mesh.dx_uniform)[0])
         summary = {"kind": config["mesh"]["kind"], "total_cells": int(mesh.cells.shape[0])}
     precision = config["runtime"]["precision"]
     if precision == "float32":
         raise CaseConfigError("the HIP path runs float64 or mixed precision")
-    geo, tables, lay = setup_objects(config, mesh)
+    geo, tables, lay = setup_objects(config, mesh, log)
     core = HipCore(mesh, device=device, precisions=("float64", "float32") if precision == "mixed" else ("float64",),
                    geometry=geo, tables=tables)
+    say("device operators uploaded")
     ph = config["physics"]
     U, Re, rho = float(ph["inflow_speed"]), float(ph["reynolds"]), float(ph["density"])
     mo = ibc["motion"]
@@ -551,6 +560,7 @@ def build_runtime(config, mesh=None, device="cuda"):
                     fail_on_cg_nonconvergence=bool(sv["fail_on_cg_nonconvergence"]),
                     cg_check_interval=int(sv["cg_check_interval"]), precision=precision,
                     mixed_inner_tol=float(sv.get("mixed_inner_tolerance", 0.1)), layout=lay)
+    say("structured grid uploaded")
     if uses_cavity(config):
         drv.set_cavity(U_lid=0.0, spanwise_zerograd=False)
     dt = float(config["time"]["dt"])

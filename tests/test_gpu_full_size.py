@@ -23,8 +23,12 @@ import pyib_hip.case as HC  # noqa: E402
 from tests.helpers import full_size_config  # noqa: E402
 
 
+def _say(tag):
+    return lambda m: print(f"[gpu] {tag}: {m}", flush=True)
+
+
 def _steps(cfg, n, mesh=None):
-    rt = HC.build_runtime(cfg, mesh=mesh)
+    rt = HC.build_runtime(cfg, mesh=mesh, log=_say(f"{cfg['name']} ({cfg['runtime']['precision']}) set-up"))
     s, s_old, vc = HC.initial_state(rt)
     rows = []
     for step in range(1, n + 1):
@@ -70,9 +74,10 @@ def test_cfg4_d40_mixed_vs_float64():
 def test_cfg5_adjoint_d16_against_finite_differences():
     steps, every = 10, 5
     cfg = full_size_config("cfg5", steps=steps)
-    rt = HC.build_runtime(cfg)
+    rt = HC.build_runtime(cfg, log=_say("adjoint D16 set-up"))
     HC.initial_state(rt)
-    obj = AD.PeriodForceObjective(rt, steps, checkpoint_every=every, tol1=1e-13, tol2=1e-12, cg_max=200000)
+    obj = AD.PeriodForceObjective(rt, steps, checkpoint_every=every, tol1=1e-13, tol2=1e-12, cg_max=200000,
+                                  log=_say("adjoint D16"))
     J, gA, gX = obj.value_and_grad()
     D = 2.0 * float(rt.config["ib"]["radius"])
     eps = 1e-6 * D
