@@ -45,6 +45,7 @@ class Grid(C.Structure):
         ("lev", Level * MAX_LEVELS), ("cell_reg", vp), ("cell_reg_g1", vp), ("edge_reg", vp), ("dir", vp),
         ("g1_nslice", C.c_int64), ("g1_row", vp), ("g1_width", vp), ("g1_sptr", vp), ("g1_col", vp), ("g1_coef", vp),
         ("v_nslice", C.c_int64), ("v_row", vp), ("v_width", vp), ("v_sptr", vp), ("v_col", vp), ("v_coef", vp),
+        ("v_rw", vp),
         ("g3_nslice", C.c_int64), ("g3_row", vp), ("g3_width", vp), ("g3_sptr", vp), ("g3_col", vp), ("g3_coef", vp),
     ]
 

@@ -86,57 +86,75 @@ __global__ void k_sell_g1(int64_t nslice, int64_t ncell, const int32_t* __restri
 }
 
 // ----------------------------------------------------------------- V
+// y(row) = rou x(row) - kv sum_j coef x[col].  With partials, block partials of
+// sum_row rw(row) x(row).y(row) (rw = the row's cell volume): these rows' share of p.A p,
+// since A = rot integr V interp curl with integr = interp^T diag(1/vol_ci) makes
+// p.A p = sum_cells vol (interp curl p).(V interp curl p).
 template <typename T>
 __global__ void k_sell_v(int64_t nslice, int64_t ncell, const int32_t* __restrict__ row,
                          const int32_t* __restrict__ width, const int64_t* __restrict__ sptr,
                          const int32_t* __restrict__ col, const T* __restrict__ coef, T rou, T kv,
-                         const T* __restrict__ x, T* __restrict__ y) {
+                         const T* __restrict__ x, T* __restrict__ y, const double* __restrict__ rw,
+                         double* __restrict__ partials) {
+  __shared__ double sh[kFinBlock / 64];
   const int lane = threadIdx.x & 63;
-  const int64_t s = wave_slice();
-  if (s >= nslice) return;
-  const int W = width[s];
-  const int64_t b = sptr[s] + lane;
-  const int r = row[s * 64 + lane];
-  T a0 = 0, a1 = 0, a2 = 0;
-  for (int j0 = 0; j0 < W; j0 += kSellBatch) {
-    int cc[kSellBatch];
-    T cf[kSellBatch], x0[kSellBatch], x1[kSellBatch], x2[kSellBatch];
+  double dot = 0.0;
+  for (int64_t s = wave_slice(); s < nslice; s += (int64_t)gridDim.x * (blockDim.x >> 6)) {
+    const int W = width[s];
+    const int64_t b = sptr[s] + lane;
+    const int r = row[s * 64 + lane];
+    T a0 = 0, a1 = 0, a2 = 0;
+    for (int j0 = 0; j0 < W; j0 += kSellBatch) {
+      int cc[kSellBatch];
+      T cf[kSellBatch], x0[kSellBatch], x1[kSellBatch], x2[kSellBatch];
 #pragma unroll
-    for (int u = 0; u < kSellBatch; ++u) {
-      if (j0 + u < W) {
-        const int64_t e = b + (int64_t)(j0 + u) * 64;
-        cc[u] = col[e];
-        cf[u] = coef[e];
+      for (int u = 0; u < kSellBatch; ++u) {
+        if (j0 + u < W) {
+          const int64_t e = b + (int64_t)(j0 + u) * 64;
+          cc[u] = col[e];
+          cf[u] = coef[e];
+        }
       }
+#pragma unroll
+      for (int u = 0; u < kSellBatch; ++u)
+        if (j0 + u < W) {
+          const int64_t o = 3 * (int64_t)cc[u];
+          x0[u] = x[o]; x1[u] = x[o + 1]; x2[u] = x[o + 2];
+        }
+#pragma unroll
+      for (int u = 0; u < kSellBatch; ++u)
+        if (j0 + u < W) { a0 += cf[u] * x0[u]; a1 += cf[u] * x1[u]; a2 += cf[u] * x2[u]; }
     }
-#pragma unroll
-    for (int u = 0; u < kSellBatch; ++u)
-      if (j0 + u < W) {
-        const int64_t o = 3 * (int64_t)cc[u];
-        x0[u] = x[o]; x1[u] = x[o + 1]; x2[u] = x[o + 2];
-      }
-#pragma unroll
-    for (int u = 0; u < kSellBatch; ++u)
-      if (j0 + u < W) { a0 += cf[u] * x0[u]; a1 += cf[u] * x1[u]; a2 += cf[u] * x2[u]; }
+    if (r >= 0) {
+      const int64_t o = 3 * (int64_t)r;
+      const T xc0 = x[o], xc1 = x[o + 1], xc2 = x[o + 2];
+      const T y0 = rou * xc0 - kv * a0, y1 = rou * xc1 - kv * a1, y2 = rou * xc2 - kv * a2;
+      y[o] = y0;
+      y[o + 1] = y1;
+      y[o + 2] = y2;
+      if (partials)
+        dot += rw[s * 64 + lane] * ((double)xc0 * (double)y0 + (double)xc1 * (double)y1 + (double)xc2 * (double)y2);
+    }
   }
-  if (r >= 0) {
-    const int64_t o = 3 * (int64_t)r;
-    y[o] = rou * x[o] - kv * a0;
-    y[o + 1] = rou * x[o + 1] - kv * a1;
-    y[o + 2] = rou * x[o + 2] - kv * a2;
+  if (partials) {
+    const double rr = block_sum(dot, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = rr;
   }
 }
 
 // ---------------------------------------------------------------- G3
-// w(row) = sum_j coef . y[col] (Dirichlet rows 0); block partials of p.w over these rows
+// w(row) = sum_j coef . y[col] (Dirichlet rows 0).  With r non-NULL: r(row) -= alpha w(row)
+// instead of storing w, and block partials of r.prec.r over these rows (as k_sg3_t)
 template <typename T>
 __global__ void k_sell_g3(int64_t nslice, int64_t ncell, const int32_t* __restrict__ row,
                           const int32_t* __restrict__ width, const int64_t* __restrict__ sptr,
                           const int32_t* __restrict__ col, const T* __restrict__ coef,
                           const uint8_t* __restrict__ mask, const T* __restrict__ y, T* __restrict__ w,
-                          const T* __restrict__ p, double* __restrict__ partials) {
+                          const double* __restrict__ scal, T* __restrict__ rv, const T* __restrict__ prec,
+                          double* __restrict__ partials) {
   __shared__ double sh[kFinBlock / 64];
   const int lane = threadIdx.x & 63;
+  const T alf = rv ? (T)scal[S_ALPHA] : T(0);
   double dot = 0.0;
   for (int64_t s = wave_slice(); s < nslice; s += (int64_t)gridDim.x * (blockDim.x >> 6)) {
     const int W = width[s];
@@ -171,13 +189,18 @@ __global__ void k_sell_g3(int64_t nslice, int64_t ncell, const int32_t* __restri
     }
     if (r >= 0) {
       if (mask && mask[r]) acc = T(0);
-      w[r] = acc;
-      if (p) dot += (double)p[r] * (double)acc;
+      if (rv) {
+        const T ri = rv[r] - alf * acc;
+        rv[r] = ri;
+        dot += (double)(ri * prec[r]) * (double)ri;
+      } else {
+        w[r] = acc;
+      }
     }
   }
   if (partials) {
-    double rr = block_sum(dot, sh);
-    if (threadIdx.x == 0) partials[blockIdx.x] = rr;
+    const double tot = block_sum(dot, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
   }
 }
 
@@ -336,82 +359,109 @@ __global__ __launch_bounds__(256) void k_sg1_t(int64_t ncell, Levels L, int ci, 
   }
 }
 
-// ---- V: y = rou x - (hdt/h^3)(a - b) on the cells of plane i (planes i-1, i, i+1 of x)
+// ---- V: y = rou x - (hdt/h^3)(a - b) on the cells of plane i (planes i-1, i, i+1 of x).
+// With partials, block partials of h^3 sum x.y over the regular cells (their share of p.A p,
+// as in k_sell_v; vol = h^3 on a regular cell)
 template <typename T>
 __global__ __launch_bounds__(256) void k_sv_t(int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
-                                              T rou, T visc, T hdt, const T* __restrict__ x, T* __restrict__ y) {
-  int bx, by, bz;
-  xcd_ids(bx, by, bz);
-  const Lev& v = L.l[bz];
-  const Tile t = tile_of(v.ny, v.nz, v.nx, ci, bx, by);
-  if (!t.live) return;
-  constexpr int PJ = TJ + 2, PK = TK + 2, PS = PJ * PK, R = 2;
-  __shared__ T sx[3][3][PS];                    // [plane slot][component][(j, k) with halo]
-  auto fetch = [&](int i, Prefetch<T, 3 * R>& f) {
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int q = threadIdx.x + r * 256;
-      const int j = t.j0 - 1 + q / PK, k = t.k0 - 1 + q % PK;
-      const bool in = q < PS && i >= 0 && i < v.nx && j >= 0 && j < v.ny && k >= 0 && k < v.nz;
-      const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) f.v[3 * r + d] = in ? x[3 * c + d] : T(0);
-    }
-  };
-  auto store = [&](int i, const Prefetch<T, 3 * R>& f) {
-    const int slot = (i + 3) % 3;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int q = threadIdx.x + r * 256;
-      if (q < PS) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) sx[slot][d][q] = f.v[3 * r + d];
-      }
-    }
-  };
-  Prefetch<T, 3 * R> f;
-  fetch(t.i0 - 1, f);
-  store(t.i0 - 1, f);
-  fetch(t.i0, f);
-  store(t.i0, f);
-  fetch(t.i0 + 1, f);
-  const T kA = (T)v.h * visc;
-  const T hv = hdt * (T)(1.0 / (v.h * v.h * v.h));
-  const int j = t.j0 + t.tj, k = t.k0 + t.tk;
-  const int q = (t.tj + 1) * PK + t.tk + 1;
-  for (int i = t.i0; i < t.i1; ++i) {
-    store(i + 1, f);
-    __syncthreads();
-    if (i + 1 < t.i1) fetch(i + 2, f);          // in flight while plane i is computed
-    const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
-    if (j < v.ny && k < v.nz && reg[c]) {
-      const int sm = (i + 2) % 3, s0 = i % 3, sp = (i + 1) % 3;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const T xc = sx[s0][d][q];
-        T a = kA * (sx[sp][d][q] - xc) + kA * (sx[s0][d][q + PK] - xc);
-        a = a + kA * (sx[s0][d][q + 1] - xc);
-        T b = kA * (xc - sx[sm][d][q]) + kA * (xc - sx[s0][d][q - PK]);
-        b = b + kA * (xc - sx[s0][d][q - 1]);
-        y[3 * c + d] = rou * xc - hv * (a - b);
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// ---- G3: w on the regular x-, y- and z-edges of plane i (planes i-1 and i of y); the
-// composite pass writes the irregular rows, empty slots are never written
-template <typename T>
-__global__ __launch_bounds__(256) void k_sg3_t(int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
-                                               const T* __restrict__ y, T* __restrict__ w, const T* __restrict__ p,
-                                               double* __restrict__ partials) {
+                                              T rou, T visc, T hdt, const T* __restrict__ x, T* __restrict__ y,
+                                              double* __restrict__ partials) {
   __shared__ double sh[kFinBlock / 64];
   int bx, by, bz;
   xcd_ids(bx, by, bz);
   const Lev& v = L.l[bz];
   const Tile t = tile_of(v.ny, v.nz, v.nx, ci, bx, by);
   double dot = 0.0;
+  if (t.live) {
+    constexpr int PJ = TJ + 2, PK = TK + 2, PS = PJ * PK, R = 2;
+    __shared__ T sx[3][3][PS];                    // [plane slot][component][(j, k) with halo]
+    auto fetch = [&](int i, Prefetch<T, 3 * R>& f) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int q = threadIdx.x + r * 256;
+        const int j = t.j0 - 1 + q / PK, k = t.k0 - 1 + q % PK;
+        const bool in = q < PS && i >= 0 && i < v.nx && j >= 0 && j < v.ny && k >= 0 && k < v.nz;
+        const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) f.v[3 * r + d] = in ? x[3 * c + d] : T(0);
+      }
+    };
+    auto store = [&](int i, const Prefetch<T, 3 * R>& f) {
+      const int slot = (i + 3) % 3;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int q = threadIdx.x + r * 256;
+        if (q < PS) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) sx[slot][d][q] = f.v[3 * r + d];
+        }
+      }
+    };
+    Prefetch<T, 3 * R> f;
+    fetch(t.i0 - 1, f);
+    store(t.i0 - 1, f);
+    fetch(t.i0, f);
+    store(t.i0, f);
+    fetch(t.i0 + 1, f);
+    const T kA = (T)v.h * visc;
+    const T hv = hdt * (T)(1.0 / (v.h * v.h * v.h));
+    const int j = t.j0 + t.tj, k = t.k0 + t.tk;
+    const int q = (t.tj + 1) * PK + t.tk + 1;
+    for (int i = t.i0; i < t.i1; ++i) {
+      store(i + 1, f);
+      __syncthreads();
+      if (i + 1 < t.i1) fetch(i + 2, f);          // in flight while plane i is computed
+      const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
+      if (j < v.ny && k < v.nz && reg[c]) {
+        const int sm = (i + 2) % 3, s0 = i % 3, sp = (i + 1) % 3;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const T xc = sx[s0][d][q];
+          T a = kA * (sx[sp][d][q] - xc) + kA * (sx[s0][d][q + PK] - xc);
+          a = a + kA * (sx[s0][d][q + 1] - xc);
+          T b = kA * (xc - sx[sm][d][q]) + kA * (xc - sx[s0][d][q - PK]);
+          b = b + kA * (xc - sx[s0][d][q - 1]);
+          const T yv = rou * xc - hv * (a - b);
+          y[3 * c + d] = yv;
+          dot += (double)xc * (double)yv;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (partials) {
+    const double r = block_sum(dot, sh);
+    if (threadIdx.x == 0)
+      partials[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = r * (v.h * v.h * v.h);
+  }
+}
+
+// ---- G3: w on the regular x-, y- and z-edges of plane i (planes i-1 and i of y); the
+// composite pass writes the irregular rows, empty slots are never written.
+// With rr non-NULL the PCG residual update is fused in: w is not stored, r -= alpha w on the
+// same rows (alpha = scal[S_ALPHA], already final since p.A p comes from the V pass) and
+// block partials of r.prec.r go to partials[block]
+template <typename T>
+__global__ __launch_bounds__(256) void k_sg3_t(int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
+                                               const T* __restrict__ y, T* __restrict__ w,
+                                               const double* __restrict__ scal, T* __restrict__ rr,
+                                               const T* __restrict__ prec, double* __restrict__ partials) {
+  __shared__ double sh[kFinBlock / 64];
+  int bx, by, bz;
+  xcd_ids(bx, by, bz);
+  const Lev& v = L.l[bz];
+  const Tile t = tile_of(v.ny, v.nz, v.nx, ci, bx, by);
+  double dot = 0.0;
+  const T alf = rr ? (T)scal[S_ALPHA] : T(0);
+  auto emit = [&](int64_t e, T val) {
+    if (rr) {
+      const T ri = rr[e] - alf * val;
+      rr[e] = ri;
+      dot += (double)(ri * prec[e]) * (double)ri;
+    } else {
+      w[e] = val;
+    }
+  };
   if (t.live) {
     constexpr int PJ = TJ + 1, PK = TK + 1, PS = PJ * PK, R = 2;   // low-side halo (j-1, k-1)
     __shared__ T sy[2][3][PS];
@@ -458,11 +508,7 @@ __global__ __launch_bounds__(256) void k_sg3_t(int64_t ncell, Levels L, int ci, 
           const T fy1 = hh * (Y(s0, 1, -1, -1) + Y(s0, 1, 0, -1));
           const T fz0 = hh * (Y(s0, 2, 0, -1) + Y(s0, 2, 0, 0));
           const T fz1 = hh * (Y(s0, 2, -1, -1) + Y(s0, 2, -1, 0));
-          if (reg[e]) {
-            const T val = (fy1 - fy0) + (fz0 - fz1);
-            w[e] = val;
-            if (p) dot += (double)p[e] * (double)val;
-          }
+          if (reg[e]) emit(e, (fy1 - fy0) + (fz0 - fz1));
         }
         {   // y-edge (i, j, k): F_x(i,j,k) - F_x(i,j,k-1) - F_z(i,j,k) + F_z(i-1,j,k)
           const int64_t e = v.e1 + ((int64_t)i * v.ny + j) * nz1 + k;
@@ -470,11 +516,7 @@ __global__ __launch_bounds__(256) void k_sg3_t(int64_t ncell, Levels L, int ci, 
           const T fx1 = hh * (Y(sm, 0, 0, -1) + Y(s0, 0, 0, -1));
           const T fz0 = hh * (Y(s0, 2, 0, -1) + Y(s0, 2, 0, 0));
           const T fz1 = hh * (Y(sm, 2, 0, -1) + Y(sm, 2, 0, 0));
-          if (reg[e]) {
-            const T val = (fx0 - fx1) + (fz1 - fz0);
-            w[e] = val;
-            if (p) dot += (double)p[e] * (double)val;
-          }
+          if (reg[e]) emit(e, (fx0 - fx1) + (fz1 - fz0));
         }
         {   // z-edge (i, j, k): -F_x(i,j,k) + F_x(i,j-1,k) + F_y(i,j,k) - F_y(i-1,j,k)
           const int64_t e = v.e2 + ((int64_t)i * ny1 + j) * v.nz + k;
@@ -482,19 +524,15 @@ __global__ __launch_bounds__(256) void k_sg3_t(int64_t ncell, Levels L, int ci, 
           const T fx1 = hh * (Y(sm, 0, -1, 0) + Y(s0, 0, -1, 0));
           const T fy0 = hh * (Y(s0, 1, -1, 0) + Y(s0, 1, 0, 0));
           const T fy1 = hh * (Y(sm, 1, -1, 0) + Y(sm, 1, 0, 0));
-          if (reg[e]) {
-            const T val = (fx1 - fx0) + (fy0 - fy1);
-            w[e] = val;
-            if (p) dot += (double)p[e] * (double)val;
-          }
+          if (reg[e]) emit(e, (fx1 - fx0) + (fy0 - fy1));
         }
       }
       __syncthreads();
     }
   }
   if (partials) {
-    double r = block_sum(dot, sh);
-    if (threadIdx.x == 0) partials[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = r;
+    const double s = block_sum(dot, sh);
+    if (threadIdx.x == 0) partials[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
   }
 }
 
@@ -519,73 +557,16 @@ __global__ void k_alpha2(const double* __restrict__ pa, int na, const double* __
 }
 
 // ------------------------------------------------------ CG on the grid
-// 16-byte vector of T (two doubles or four floats) for the streaming CG updates
-template <typename T> struct V16;
-template <> struct V16<double> { using type = double2; static constexpr int n = 2; };
-template <> struct V16<float> { using type = float4; static constexpr int n = 4; };
-template <typename T>
-union Pack {
-  typename V16<T>::type v;
-  T a[V16<T>::n];
-};
-
-// x += alpha p, r -= alpha w, block partials of r.prec.r; 16-byte loads and stores, each
-// thread kPerThread vectors per chunk (the tail of n is done element by element)
-template <typename T>
-__global__ void k_gcg_update(int64_t n, const double* __restrict__ scal, T* __restrict__ x, T* __restrict__ r,
-                             const T* __restrict__ p, const T* __restrict__ w, const T* __restrict__ prec,
-                             double* __restrict__ partials) {
-  __shared__ double sh[kFinBlock / 64];
-  using VT = typename V16<T>::type;
-  constexpr int NV = V16<T>::n;
-  const T alf = (T)scal[S_ALPHA];
-  double acc = 0.0;
-  const int64_t nvec = n / NV;
-  const int64_t chunk = (int64_t)blockDim.x * kPerThread;
-  VT* xv = reinterpret_cast<VT*>(x);
-  VT* rv = reinterpret_cast<VT*>(r);
-  const VT* pv = reinterpret_cast<const VT*>(p);
-  const VT* wv = reinterpret_cast<const VT*>(w);
-  const VT* mv = reinterpret_cast<const VT*>(prec);
-  for (int64_t base = blockIdx.x * chunk; base < nvec; base += (int64_t)gridDim.x * chunk) {
-    Pack<T> X[kPerThread], P[kPerThread], R[kPerThread], W[kPerThread], M[kPerThread];
-#pragma unroll
-    for (int u = 0; u < kPerThread; ++u) {
-      const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
-      if (i < nvec) { X[u].v = xv[i]; P[u].v = pv[i]; R[u].v = rv[i]; W[u].v = wv[i]; M[u].v = mv[i]; }
-    }
-#pragma unroll
-    for (int u = 0; u < kPerThread; ++u) {
-      const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
-      if (i < nvec) {
-#pragma unroll
-        for (int e = 0; e < NV; ++e) {
-          X[u].a[e] = X[u].a[e] + alf * P[u].a[e];
-          const T ri = R[u].a[e] - alf * W[u].a[e];
-          R[u].a[e] = ri;
-          acc += (double)(ri * M[u].a[e]) * (double)ri;
-        }
-        xv[i] = X[u].v;
-        rv[i] = R[u].v;
-      }
-    }
-  }
-  if (blockIdx.x == 0) {
-    for (int64_t i = nvec * NV + threadIdx.x; i < n; i += blockDim.x) {
-      x[i] = x[i] + alf * p[i];
-      const T ri = r[i] - alf * w[i];
-      r[i] = ri;
-      acc += (double)(ri * prec[i]) * (double)ri;
-    }
-  }
-  double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) partials[blockIdx.x] = s;
-}
-__global__ void k_gcg_beta(const double* __restrict__ partials, int nb, double* __restrict__ scal) {
+// eta_new = r.prec.r from the two partial rows of the fused G3 passes, beta = eta_new / eta
+__global__ void k_gcg_beta(const double* __restrict__ pa, int na, const double* __restrict__ pb, int nb,
+                           double* __restrict__ scal) {
   __shared__ double sh[kFinBlock / 64];
   double acc = 0.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partials[b];
-  double en = block_sum(acc, sh);
+  for (int b = threadIdx.x; b < na; b += blockDim.x) acc += pa[b];
+  const double ea = block_sum(acc, sh);
+  acc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += pb[b];
+  const double en = ea + block_sum(acc, sh);
   if (threadIdx.x == 0) {
     scal[S_ETA_NEW] = en;
     const double eta = scal[S_ETA];
@@ -598,6 +579,24 @@ __global__ void k_gcg_dir(int64_t n, const double* __restrict__ scal, const T* _
   const T beta = (T)scal[S_BETA];
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = r[i] * prec[i] + beta * p[i];
+}
+// x += alpha p (alpha of this iteration), then p = r*prec + beta*p
+template <typename T>
+__global__ void k_gcg_dir_x(int64_t n, const double* __restrict__ scal, const T* __restrict__ r,
+                            const T* __restrict__ prec, T* __restrict__ p, T* __restrict__ x) {
+  const T alf = (T)scal[S_ALPHA], beta = (T)scal[S_BETA];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const T pi = p[i];
+    x[i] = x[i] + alf * pi;
+    p[i] = r[i] * prec[i] + beta * pi;
+  }
+}
+// x += alpha p alone (the last iteration before the host's stop test)
+template <typename T>
+__global__ void k_gcg_x(int64_t n, const double* __restrict__ scal, const T* __restrict__ p, T* __restrict__ x) {
+  const T alf = (T)scal[S_ALPHA];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = x[i] + alf * p[i];
 }
 __global__ void k_gcg_shift(double* __restrict__ scal) { scal[S_ETA] = scal[S_ETA_NEW]; }
 // residual replacement: eta_new <- the replaced residual's eta, beta from it
@@ -674,13 +673,21 @@ __global__ void k_permute(int64_t n, const int64_t* __restrict__ idx, const int8
 
 // ------------------------------------------------------- launchers
 inline int slice_grid(int64_t nslice) { return grid_for(nslice * 64); }
-inline int slice_red_grid(int64_t nslice) { return red_grid_for(nslice * 64); }
-// w = A p on the grid; the p.w partials go to ws->partials [0, nb1) and [kRedGrid, kRedGrid+nb2).
-// Each pass: the tiled stencil writes the regular rows, the composite kernel the others
-// (disjoint rows).  Running the two on separate streams measured no faster than in order.
+// one wavefront per slice, capped at the partials a reduction holds (the kernels stride over
+// the slices)
+inline int slice_red_grid(int64_t nslice) {
+  const int g = slice_grid(nslice);
+  return g > kRedGrid ? kRedGrid : g;
+}
+// G1 and V passes of A p: x = interp curl p into ws->cell_a, V x into ws->cell_b.  With
+// want_dot, partials of p.A p go to ws->partials [0, *nv) (composite V rows) and
+// [kRedGrid, kRedGrid + *nt) (tiled V pass): p.A p = sum_cells vol x.(V x), so the dot needs
+// no second read of p.  It equals p.w when p is zero on the Dirichlet rows, as every PCG
+// direction is.  Each pass: the tiled stencil writes the regular rows, the composite kernel
+// the others (disjoint rows).  Running the two on separate streams measured no faster.
 template <typename T>
-int gflow(const pyib_grid* g, const pyib_work* ws, double rou, double visc, double hdt, const T* p, T* w,
-          bool want_dot, int* nb1, int* nb2, hipStream_t st) {
+void g1_v(const pyib_grid* g, const pyib_work* ws, double rou, double visc, double hdt, const T* p, bool want_dot,
+          int* nv, int* nt, hipStream_t st) {
   const Levels L = levels_of(g);
   T* ca = static_cast<T*>(ws->cell_a);
   T* cb = static_cast<T*>(ws->cell_b);
@@ -690,44 +697,70 @@ int gflow(const pyib_grid* g, const pyib_work* ws, double rou, double visc, doub
   if (g->g1_nslice > 0)
     k_sell_g1<T><<<slice_grid(g->g1_nslice), kBlock, 0, st>>>(g->g1_nslice, g->ncell, g->g1_row, g->g1_width,
                                                              g->g1_sptr, g->g1_col, (const T*)g->g1_coef, p, ca);
-  k_sv_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, g->cell_reg, (T)rou, (T)visc, (T)hdt, ca, cb);
+  const int n1 = slice_red_grid(g->v_nslice);
+  k_sv_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, g->cell_reg, (T)rou, (T)visc, (T)hdt, ca, cb,
+                                   want_dot ? ws->partials + kRedGrid : nullptr);
   if (g->v_nslice > 0)
-    k_sell_v<T><<<slice_grid(g->v_nslice), kBlock, 0, st>>>(g->v_nslice, g->ncell, g->v_row, g->v_width, g->v_sptr,
-                                                           g->v_col, (const T*)g->v_coef, (T)rou, (T)(hdt * visc),
-                                                           ca, cb);
-  const int n1 = slice_red_grid(g->g3_nslice), n2 = (int)(gt.x * gt.y * gt.z);
-  k_sg3_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, g->edge_reg, cb, w, want_dot ? p : nullptr,
-                                    want_dot ? ws->partials + kRedGrid : nullptr);
+    k_sell_v<T><<<n1, kBlock, 0, st>>>(g->v_nslice, g->ncell, g->v_row, g->v_width, g->v_sptr, g->v_col,
+                                       (const T*)g->v_coef, (T)rou, (T)(hdt * visc), ca, cb, g->v_rw,
+                                       want_dot ? ws->partials : nullptr);
+  *nv = g->v_nslice > 0 ? n1 : 0;
+  *nt = (int)(gt.x * gt.y * gt.z);
+}
+
+// G3 pass from ws->cell_b: w = rot integr y, or with rr non-NULL the fused residual update
+// rr -= alpha w with r.prec.r partials in ws->partials [2 kRedGrid, + *nt) (tiled) and
+// [3 kRedGrid, + *ns) (composite rows)
+template <typename T>
+void g3(const pyib_grid* g, const pyib_work* ws, T* w, const double* scal, T* rr, const T* prec, int* nt, int* ns,
+        hipStream_t st) {
+  const Levels L = levels_of(g);
+  const T* cb = static_cast<const T*>(ws->cell_b);
+  int ci = CI;
+  const dim3 gt = tile_grid(L, &ci);
+  const int n3 = slice_red_grid(g->g3_nslice);
+  k_sg3_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, g->edge_reg, cb, w, scal, rr, prec,
+                                    rr ? ws->partials + 2 * kRedGrid : nullptr);
   if (g->g3_nslice > 0)
-    k_sell_g3<T><<<n1, kBlock, 0, st>>>(g->g3_nslice, g->ncell, g->g3_row, g->g3_width, g->g3_sptr, g->g3_col,
-                                        (const T*)g->g3_coef, g->dir, cb, w, want_dot ? p : nullptr,
-                                        want_dot ? ws->partials : nullptr);
-  *nb1 = g->g3_nslice > 0 ? n1 : 0;
-  *nb2 = n2;
+    k_sell_g3<T><<<n3, kBlock, 0, st>>>(g->g3_nslice, g->ncell, g->g3_row, g->g3_width, g->g3_sptr, g->g3_col,
+                                        (const T*)g->g3_coef, g->dir, cb, w, scal, rr, prec,
+                                        rr ? ws->partials + 3 * kRedGrid : nullptr);
+  if (nt) *nt = (int)(gt.x * gt.y * gt.z);
+  if (ns) *ns = g->g3_nslice > 0 ? n3 : 0;
+}
+
+// w = A p on the grid (see g1_v for the dot partials)
+template <typename T>
+int gflow(const pyib_grid* g, const pyib_work* ws, double rou, double visc, double hdt, const T* p, T* w,
+          bool want_dot, int* nb1, int* nb2, hipStream_t st) {
+  g1_v<T>(g, ws, rou, visc, hdt, p, want_dot, nb1, nb2, st);
+  g3<T>(g, ws, w, nullptr, nullptr, nullptr, nullptr, nullptr, st);
   return 0;
 }
 
+// n_iter PCG iterations.  Per iteration: G1, V (+ p.A p partials), alpha, G3 fused with
+// r -= alpha w (+ r.prec.r partials), beta, then x += alpha p with p = r prec + beta p in
+// one pass (or x alone before the host's stop test when skip_last)
 template <typename T>
 int gcg_iterate(const pyib_grid* g, const pyib_work* ws, const pyib_cg* cg, double rou, double visc, double hdt,
                 int n_iter, int skip_last, hipStream_t st) {
   T* x = static_cast<T*>(cg->x);
   T* r = static_cast<T*>(cg->r);
   T* p = static_cast<T*>(cg->p);
-  T* w = static_cast<T*>(cg->w);
   const T* prec = static_cast<const T*>(cg->prec);
   const int64_t n = g->nedge;
-  const int nb = red_grid_for(n / V16<T>::n + 1);   // the update streams 16-byte vectors
-  // the update's partials use a third row so they never alias the dot of A p
-  double* upart = ws->partials + 2 * kRedGrid;
   for (int it = 0; it < n_iter; ++it) {
-    int nb1, nb2;
-    gflow<T>(g, ws, rou, visc, hdt, p, w, true, &nb1, &nb2, st);
-    k_alpha2<<<1, kFinBlock, 0, st>>>(ws->partials, nb1, ws->partials + kRedGrid, nb2, cg->scal, 1, nullptr);
-    k_gcg_update<T><<<nb, kBlock, 0, st>>>(n, cg->scal, x, r, p, w, prec, upart);
-    k_gcg_beta<<<1, kFinBlock, 0, st>>>(upart, nb, cg->scal);
+    int nv, nt, nt3, ns3;
+    g1_v<T>(g, ws, rou, visc, hdt, p, true, &nv, &nt, st);
+    k_alpha2<<<1, kFinBlock, 0, st>>>(ws->partials, nv, ws->partials + kRedGrid, nt, cg->scal, 1, nullptr);
+    g3<T>(g, ws, nullptr, cg->scal, r, prec, &nt3, &ns3, st);
+    k_gcg_beta<<<1, kFinBlock, 0, st>>>(ws->partials + 2 * kRedGrid, nt3, ws->partials + 3 * kRedGrid, ns3,
+                                        cg->scal);
     if (!(skip_last && it == n_iter - 1)) {
-      k_gcg_dir<T><<<grid_for(n), kBlock, 0, st>>>(n, cg->scal, r, prec, p);
+      k_gcg_dir_x<T><<<grid_for(n), kBlock, 0, st>>>(n, cg->scal, r, prec, p, x);
       k_gcg_shift<<<1, 1, 0, st>>>(cg->scal);
+    } else {
+      k_gcg_x<T><<<grid_for(n), kBlock, 0, st>>>(n, cg->scal, p, x);   // x current for the host's test
     }
   }
   return check_launch("pyib_grid_cg_iterate");

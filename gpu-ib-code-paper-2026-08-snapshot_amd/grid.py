@@ -16,7 +16,7 @@ import torch
 
 from . import _native as N
 from .core import dev_stream, ptr
-from .layout import MAX_LEVELS, build_layout, dir_mask_internal, to_sell
+from .layout import MAX_LEVELS, build_layout, dir_mask_internal, row_volumes, to_sell
 
 
 class HipGrid:
@@ -42,6 +42,8 @@ class HipGrid:
             self._comp[name] = {"rows": t(c.rows, torch.int32), "width": t(c.width, torch.int32),
                                 "sptr": t(c.sptr, torch.int64), "col": t(c.col, torch.int32),
                                 "n": int(c.width.shape[0]), "coef64": c.coef}
+            if name == "vis":
+                self._comp[name]["rw"] = t(row_volumes(lay, core.geo, c.rows), torch.float64)
         self._coef = {}
         self._struct = {}
         self.work = {}
@@ -87,6 +89,7 @@ class HipGrid:
             setattr(g, pre + "_sptr", ptr(cc["sptr"]))
             setattr(g, pre + "_col", ptr(cc["col"]))
             setattr(g, pre + "_coef", ptr(k[name]))
+        g.v_rw = ptr(c["vis"]["rw"])
         return g
 
     def set_dir_mask(self, dir_mask_ref):
@@ -117,12 +120,15 @@ class HipGrid:
         v[self.lay.edge_map] = self.lay.edge_sigma * np.asarray(x_ref)
         return torch.from_numpy(v).to(self.device)
 
-    def flow_mat(self, p_int, rou, visc, dt, out=None):
+    def flow_mat(self, p_int, rou, visc, dt, out=None, with_dot=False):
+        """w = A p on the internal numbering; with ``with_dot`` also p.A p (float64, device)
+        as the PCG forms it from the cell passes."""
         prec = "float64" if p_int.dtype == torch.float64 else "float32"
         out = torch.zeros(self.nedge, dtype=p_int.dtype, device=self.device) if out is None else out
+        dot = torch.zeros(1, dtype=torch.float64, device=self.device) if with_dot else None
         N.call("pyib_grid_flow_mat", self.struct(prec), self.work_struct(prec), float(rou), float(visc),
-               0.5 * float(dt), ptr(p_int), ptr(out), C.c_void_p(0), dev_stream())
-        return out
+               0.5 * float(dt), ptr(p_int), ptr(out), ptr(dot) if with_dot else C.c_void_p(0), dev_stream())
+        return (out, dot) if with_dot else out
 
 
 def _code(dtype):

@@ -428,6 +428,19 @@ def to_sell(comp, C=SLICE, sigma=8 * SLICE):
     return SellComposite(rows, sw.astype(np.int32), sptr, col, coef)
 
 
+def row_volumes(lay, geo, rows):
+    """Cell volume 1/vol_ci of each internal cell row (0 for -1 padding and for cells with
+    vol_ci <= 0).  The composite V kernel weighs each row's x.(V x) by it to form its share
+    of p.A p: A = rot integr V interp curl with integr = interp^T diag(1/vol_ci)
+    (ib_sparse.py:29-50), so p.A p = sum_cells vol x.(V x) with x = interp curl p."""
+    vol = np.zeros(lay.ncell)
+    vci = np.asarray(geo.vol_ci, dtype=np.float64)
+    pos = vci > 0
+    vol[lay.cell_map[pos]] = 1.0 / vci[pos]
+    rows = np.asarray(rows)
+    return np.where(rows >= 0, vol[np.maximum(rows, 0)], 0.0)
+
+
 def sell_apply(sc, x_gather, C=SLICE):
     """Host emulation of a sliced-ELL row sum: out[row] = sum_j coef * x_gather(col), summed
     sequentially in entry order as the kernels do (padding entries add +0)."""

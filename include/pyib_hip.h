@@ -114,6 +114,7 @@ typedef struct pyib_grid {
   const int32_t *g1_col; const void *g1_coef;   /* coef entries x 3 */
   int64_t v_nslice;  const int32_t *v_row, *v_width;  const int64_t *v_sptr;
   const int32_t *v_col;  const void *v_coef;    /* coef entries */
+  const double *v_rw;   /* 64*v_nslice: volume (1/vol_ci) of the row's cell, 0 on padding */
   int64_t g3_nslice; const int32_t *g3_row, *g3_width; const int64_t *g3_sptr;
   const int32_t *g3_col; const void *g3_coef;   /* coef entries x 3 */
 } pyib_grid;
@@ -168,14 +169,21 @@ int pyib_extrapolate(int64_t n, const double *s, const double *s_old, double dt,
                      double *out, void *stream);
 
 /* ---- the same CG on the structured layout (ib_driver_np.py:311-410) ----
- * Workspace cell_a/cell_b hold 3*ncell (internal) values; partials >= 3*PYIB_MAX_PARTIALS. */
+ * Workspace cell_a/cell_b hold 3*ncell (internal) values; partials >= 4*PYIB_MAX_PARTIALS.
+ * flow_mat: w = A p (0 on dir).  If dot_out is non-NULL, *dot_out (device) receives
+ * p.A p = sum_cells vol x.(V x), x = interp(curl p), in double (deterministic): p.w when
+ * p is zero on the dir rows. */
 int pyib_grid_flow_mat(const pyib_grid *g, const pyib_work *ws, double rou, double visc, double hdt,
                        const void *p, void *w, double *dot_out, void *stream);
 int pyib_grid_cg_begin(const pyib_grid *g, const pyib_work *ws, const pyib_cg *cg, const void *res,
                        const void *s, double tol1, double tol2, void *stream);
+/* n_iter iterations as pyib_cg_iterate.  cg->w is not used: r -= alpha A p is fused into the
+ * last operator pass, and x += alpha p into the direction update (done alone when the last
+ * iteration skips the direction, so x is current for the host's stop test). */
 int pyib_grid_cg_iterate(const pyib_grid *g, const pyib_work *ws, const pyib_cg *cg, double rou,
                          double visc, double hdt, int32_t n_iter, int32_t skip_last_direction,
                          void *stream);
+/* p = r*prec + beta*p; eta = eta_new */
 int pyib_grid_cg_direction(const pyib_grid *g, const pyib_cg *cg, void *stream);
 int pyib_grid_cg_true_residual(const pyib_grid *g, const pyib_work *ws, const pyib_cg *cg,
                                const void *res, const void *s, void *tmp, void *tr, double rou,

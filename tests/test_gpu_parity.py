@@ -136,11 +136,19 @@ def test_grid_flow_mat(h, boxes):
     p = rng.standard_normal(oc.nedge)
     kw = dict(visc=0.45, rou=1.0, dt=0.0043633)
     ref = oc.flow_mat(p.copy(), alpd=0.5, **kw)
-    w = g.to_reference(g.flow_mat(g.to_internal(T(p)), kw["rou"], kw["visc"], kw["dt"]))
+    p_int = g.to_internal(T(p))
+    w_int, dot = g.flow_mat(p_int, kw["rou"], kw["visc"], kw["dt"], with_dot=True)
+    w = g.to_reference(w_int)
     assert rel(H(w), ref) < 1e-13
-    w32 = g.to_reference(g.flow_mat(g.to_internal(T(p), dtype=torch.float32), kw["rou"], kw["visc"], kw["dt"]),
-                         dtype=torch.float64)
+    # the PCG's p.A p, formed from the cell passes, against p.w and the oracle's p.(A p)
+    pap = float(p @ ref)
+    assert abs(float(dot.item()) - pap) <= 1e-12 * abs(pap)
+    assert abs(float(dot.item()) - float((p_int * w_int).sum())) <= 1e-12 * abs(pap)
+    p32 = g.to_internal(T(p), dtype=torch.float32)
+    w32i, dot32 = g.flow_mat(p32, kw["rou"], kw["visc"], kw["dt"], with_dot=True)
+    w32 = g.to_reference(w32i, dtype=torch.float64)
     assert rel(H(w32), ref) < 1e-5
+    assert abs(float(dot32.item()) - pap) <= 1e-5 * abs(pap)
 
 
 def _run_hip(cfg, steps):

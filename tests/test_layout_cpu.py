@@ -11,7 +11,7 @@ import pytest
 import pyib_hip  # noqa: F401
 import pyib_hip.mesh as PM
 from pyib_hip.geometry import Geometry
-from pyib_hip.layout import build_layout, to_internal_edges, to_reference_edges
+from pyib_hip.layout import build_layout, row_volumes, to_internal_edges, to_reference_edges
 from pyib_hip.tables import build_tables
 from oracle.pyib_ref import core as OCore
 from tests.helpers import rel
@@ -22,8 +22,9 @@ def blocks(lay, lv, v):
     return [v[lv.e_off[d]:lv.e_off[d] + int(np.prod(shp[d]))].reshape(shp[d]) for d in range(3)]
 
 
-def emulate(lay, p, rou, visc, hdt):
-    """Structured stencils on regular rows + composite rows elsewhere (the kernels' arithmetic)."""
+def emulate(lay, p, rou, visc, hdt, cells=False):
+    """Structured stencils on regular rows + composite rows elsewhere (the kernels' arithmetic);
+    with ``cells`` also the intermediate cell vectors x = interp curl p and V x."""
     vc = np.zeros((3, lay.ncell))
     for lv in lay.levels:
         px, py, pz = blocks(lay, lv, p)
@@ -81,7 +82,7 @@ def emulate(lay, p, rou, visc, hdt):
     for r in range(g3.rows.shape[0]):
         a, b = g3.ptr[r], g3.ptr[r + 1]
         w[g3.rows[r]] = (g3.coef[a:b] * v2[:, g3.col[a:b]].T).sum()
-    return w
+    return (w, vc, v2) if cells else w
 
 
 @pytest.mark.parametrize("h,boxes", [(0.0625, "nested3"), (0.5 / 24, "nested2"), (0.125, "wake")])
@@ -132,3 +133,30 @@ def test_sliced_ell_keeps_every_row_sum():
             a, b = comp.ptr[k], comp.ptr[k + 1]
             terms = comp.coef[a:b] * (x[comp.col[a:b]][:, None] if comp.coef.ndim == 2 else x[comp.col[a:b]])
             assert np.array_equal(got[int(r)], np.cumsum(terms, axis=0)[-1])      # sequential, as the kernels
+
+
+@pytest.mark.parametrize("h,boxes", [(0.0625, "nested3"), (0.125, "wake")])
+def test_quadratic_form_from_cell_passes(h, boxes):
+    """p.A p as the V passes form it (h^3 x.(V x) on regular cells, row_volumes-weighted on
+    the composite rows) equals p.w, and the oracle's p.(A p), when p is zero off the real
+    edges; a wrong weight (vol_ci instead of its inverse) or a cell left out breaks it."""
+    bx = {"nested3": PM.nested_box_boxes([1.5, 1.25, 1.25], 3), "wake": PM.wake_boxes()}[boxes]
+    m = PM.make_nested_mesh(h, bx)
+    g = Geometry(m)
+    lay = build_layout(m, g, build_tables(g))
+    rng = np.random.default_rng(11)
+    p_ref = rng.standard_normal(g.nedge)
+    p = to_internal_edges(lay, p_ref)
+    rou, visc, dt = 1.0, 0.45, 0.004
+    w, vc, y = emulate(lay, p, rou, visc, 0.5 * dt, cells=True)
+    wt = np.zeros(lay.ncell)
+    for lv in lay.levels:
+        sl = slice(lv.cell_off, lv.cell_off + lv.ncell)
+        wt[sl][lay.reg_cell[sl].astype(bool)] = lv.h ** 3
+    wt[lay.vis.rows] = row_volumes(lay, g, lay.vis.rows)
+    quad = float((wt * (vc * y).sum(axis=0)).sum())
+    pw = float(p @ w)
+    ref = float(p_ref @ OCore.Core(m).flow_mat(p_ref.copy(), alpd=0.5, visc=visc, rou=rou, dt=dt))
+    assert abs(quad - pw) <= 1e-13 * abs(pw)
+    assert abs(quad - ref) <= 1e-13 * abs(ref)
+    assert np.all(row_volumes(lay, g, np.array([-1])) == 0.0)
