@@ -43,6 +43,7 @@ class Grid(C.Structure):
     _fields_ = [
         ("dtype", C.c_int32), ("nlevels", C.c_int32), ("ncell", C.c_int64), ("nedge", C.c_int64),
         ("lev", Level * MAX_LEVELS), ("cell_reg", vp), ("cell_reg_g1", vp), ("edge_reg", vp), ("dir", vp),
+        ("cell_g1v", vp),
         ("g1_nslice", C.c_int64), ("g1_row", vp), ("g1_width", vp), ("g1_sptr", vp), ("g1_col", vp), ("g1_coef", vp),
         ("v_nslice", C.c_int64), ("v_row", vp), ("v_width", vp), ("v_sptr", vp), ("v_col", vp), ("v_coef", vp),
         ("v_rw", vp),

@@ -359,6 +359,175 @@ __global__ __launch_bounds__(256) void k_sg1_t(int64_t ncell, Levels L, int ci, 
   }
 }
 
+// ---- G1 + V fused: y = V x with x = interp(curl p) formed in LDS, never read back from HBM.
+// Each step computes x on plane ip over the tile and a one-cell halo in j and k (10 x 34
+// cells) from the x-edges of plane ip and the y/z-edges of planes ip and ip+1, then applies
+// V on plane ip-1 from the x planes ip-2 (kept in registers per cell), ip-1 and ip (LDS).
+// Cell flags g1v: bit 0 regular G1 row, bit 1 store x, bit 2 regular V row.  x of a cell comes
+// from: the stencil (bit 0), HBM (in the level box but not a regular G1 row: written by the
+// composite G1 rows, or zero in a hole), or 0 outside the box.  x of an owned regular cell is
+// stored only when bit 1 is set (the composite V rows read it).  The flags of the region are
+// staged with the edges, so the V step reads its own cell's flag from LDS.  With partials:
+// block partials of h^3 sum x.y over the regular V cells (their share of p.A p).
+template <typename T>
+__global__ __launch_bounds__(256) void k_sg1v_t(int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ g1v,
+                                                T rou, T visc, T hdt,
+                                                const T* __restrict__ p, T* __restrict__ x, T* __restrict__ y,
+                                                double* __restrict__ partials) {
+  __shared__ double sh[kFinBlock / 64];
+  int bx, by, bz;
+  xcd_ids(bx, by, bz);
+  const Lev& v = L.l[bz];
+  const Tile t = tile_of(v.ny, v.nz, v.nx, ci, bx, by);
+  double dot = 0.0;
+  if (t.live) {
+    constexpr int RJ = TJ + 2, RK = TK + 2, RS = RJ * RK;   // cells with the halo: j0-1.., k0-1..
+    constexpr int XJ = RJ + 1, XK = RK + 1;                 // x-edges: rows j..j+1, k..k+1
+    constexpr int YJ = RJ, YK = RK + 1;                     // y-edges: k..k+1
+    constexpr int ZJ = RJ + 1, ZK = RK;                     // z-edges: j..j+1
+    constexpr int R = 2;                                    // every staged plane has < 2*256 entries
+    __shared__ T sx[XJ * XK];
+    __shared__ T sy[2][YJ * YK];
+    __shared__ T sz[2][ZJ * ZK];
+    __shared__ T sc[2][3][RS];
+    __shared__ uint8_t sf[2][RS];
+    const int ny1 = v.ny + 1, nz1 = v.nz + 1;
+    const int jb = t.j0 - 1, kb = t.k0 - 1;
+    auto slot = [](int i) { return (i + 2) & 1; };         // i >= -1
+    // loads of step ip: x-edges and cell flags of plane ip, y/z-edges of plane ip+1
+    auto fetch = [&](int ip, Prefetch<T, R>& fx, Prefetch<T, R>& fy, Prefetch<T, R>& fz, uint8_t* fl) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int q = threadIdx.x + r * 256;
+        {
+          const int j = jb + q / XK, k = kb + q % XK;
+          const bool in = q < XJ * XK && ip >= 0 && ip < v.nx && j >= 0 && j < ny1 && k >= 0 && k < nz1;
+          fx.v[r] = in ? p[v.e0 + ((int64_t)ip * ny1 + j) * nz1 + k] : T(0);
+        }
+        {
+          const int j = jb + q / YK, k = kb + q % YK;
+          const bool in = q < YJ * YK && ip + 1 >= 0 && ip + 1 <= v.nx && j >= 0 && j < v.ny && k >= 0 && k < nz1;
+          fy.v[r] = in ? p[v.e1 + ((int64_t)(ip + 1) * v.ny + j) * nz1 + k] : T(0);
+        }
+        {
+          const int j = jb + q / ZK, k = kb + q % ZK;
+          const bool in = q < ZJ * ZK && ip + 1 >= 0 && ip + 1 <= v.nx && j >= 0 && j < ny1 && k >= 0 && k < v.nz;
+          fz.v[r] = in ? p[v.e2 + ((int64_t)(ip + 1) * ny1 + j) * v.nz + k] : T(0);
+        }
+        {
+          const int j = jb + q / RK, k = kb + q % RK;
+          const bool in = q < RS && ip >= 0 && ip < v.nx && j >= 0 && j < v.ny && k >= 0 && k < v.nz;
+          fl[r] = in ? (uint8_t)(0x80 | g1v[v.cell_off + ((int64_t)ip * v.ny + j) * v.nz + k]) : (uint8_t)0;
+        }
+      }
+    };
+    // y/z-edges of plane i0-1
+    {
+      const int ip = t.i0 - 1;
+      for (int q = threadIdx.x; q < YJ * YK; q += blockDim.x) {
+        const int j = jb + q / YK, k = kb + q % YK;
+        const bool in = ip >= 0 && ip <= v.nx && j >= 0 && j < v.ny && k >= 0 && k < nz1;
+        sy[slot(ip)][q] = in ? p[v.e1 + ((int64_t)ip * v.ny + j) * nz1 + k] : T(0);
+      }
+      for (int q = threadIdx.x; q < ZJ * ZK; q += blockDim.x) {
+        const int j = jb + q / ZK, k = kb + q % ZK;
+        const bool in = ip >= 0 && ip <= v.nx && j >= 0 && j < ny1 && k >= 0 && k < v.nz;
+        sz[slot(ip)][q] = in ? p[v.e2 + ((int64_t)ip * ny1 + j) * v.nz + k] : T(0);
+      }
+    }
+    Prefetch<T, R> fx, fy, fz;
+    uint8_t fl[R];
+    fetch(t.i0 - 1, fx, fy, fz, fl);
+    const T cf = (T)(1.0 / (2.0 * v.h * v.h));
+    const T kA = (T)v.h * visc;
+    const T hv = hdt * (T)(1.0 / (v.h * v.h * v.h));
+    const int j = t.j0 + t.tj, k = t.k0 + t.tk;
+    const int qo = (t.tj + 1) * RK + t.tk + 1;              // this thread's cell in the region
+    T xm[3] = {T(0), T(0), T(0)};                            // x of plane ip-2 at this cell
+    for (int ip = t.i0 - 1; ip <= t.i1; ++ip) {
+      const int a = slot(ip), b = slot(ip + 1);
+      uint8_t fc[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int q = threadIdx.x + r * 256;
+        if (q < XJ * XK) sx[q] = fx.v[r];
+        if (q < YJ * YK) sy[b][q] = fy.v[r];
+        if (q < ZJ * ZK) sz[b][q] = fz.v[r];
+        fc[r] = fl[r];
+      }
+      __syncthreads();
+      if (ip + 1 <= t.i1) fetch(ip + 1, fx, fy, fz, fl);   // in flight while plane ip is computed
+#pragma unroll 1
+      for (int r = 0; r < R; ++r) {                          // rolled: fewer live registers
+        const int q = threadIdx.x + r * 256;
+        if (q < RS) {
+          const int jj = q / RK, kk = q % RK;
+          T x0 = T(0), x1 = T(0), x2 = T(0);
+          const int64_t c = v.cell_off + ((int64_t)ip * v.ny + (jb + jj)) * v.nz + (kb + kk);
+          const uint8_t fr = r ? fc[1] : fc[0];
+          if (fr & 1) {
+            auto PX = [&](int dj, int dk) { return sx[(jj + dj) * XK + kk + dk]; };
+            auto PY = [&](int s, int dk) { return sy[s][jj * YK + kk + dk]; };
+            auto PZ = [&](int s, int dj) { return sz[s][(jj + dj) * ZK + kk]; };
+            const T ux0 = (PY(a, 0) - PY(a, 1)) + (PZ(a, 1) - PZ(a, 0));
+            const T ux1 = (PY(b, 0) - PY(b, 1)) + (PZ(b, 1) - PZ(b, 0));
+            const T uy0 = (PX(0, 1) - PX(0, 0)) - (PZ(b, 0) - PZ(a, 0));
+            const T uy1 = (PX(1, 1) - PX(1, 0)) - (PZ(b, 1) - PZ(a, 1));
+            const T uz0 = (PY(b, 0) - PY(a, 0)) - (PX(1, 0) - PX(0, 0));
+            const T uz1 = (PY(b, 1) - PY(a, 1)) - (PX(1, 1) - PX(0, 1));
+            x0 = (ux0 + ux1) * cf;
+            x1 = (uy0 + uy1) * cf;
+            x2 = (uz0 + uz1) * cf;
+            const bool own = jj >= 1 && jj <= TJ && kk >= 1 && kk <= TK && ip >= t.i0 && ip < t.i1;
+            if ((fr & 2) && own) {
+              x[3 * c] = x0;
+              x[3 * c + 1] = x1;
+              x[3 * c + 2] = x2;
+            }
+          } else if (fr & 0x80) {
+            x0 = x[3 * c];
+            x1 = x[3 * c + 1];
+            x2 = x[3 * c + 2];
+          }
+          sc[a][0][q] = x0;
+          sc[a][1][q] = x1;
+          sc[a][2][q] = x2;
+          sf[a][q] = fr;
+        }
+      }
+      __syncthreads();
+      const int i = ip - 1;                                  // V on plane i
+      if (i >= t.i0) {
+        const int s0 = slot(i);
+        if (sf[s0][qo] & 4) {
+          const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const T xc = sc[s0][d][qo];
+            T aa = kA * (sc[a][d][qo] - xc) + kA * (sc[s0][d][qo + RK] - xc);
+            aa = aa + kA * (sc[s0][d][qo + 1] - xc);
+            T bb = kA * (xc - xm[d]) + kA * (xc - sc[s0][d][qo - RK]);
+            bb = bb + kA * (xc - sc[s0][d][qo - 1]);
+            const T yv = rou * xc - hv * (aa - bb);
+            y[3 * c + d] = yv;
+            dot += (double)xc * (double)yv;
+          }
+        }
+      }
+      if (ip >= t.i0) {
+        const int s0 = slot(ip - 1);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) xm[d] = sc[s0][d][qo];
+      }
+    }
+  }
+  if (partials) {
+    const double r = block_sum(dot, sh);
+    if (threadIdx.x == 0)
+      partials[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = r * (v.h * v.h * v.h);
+  }
+}
+
 // ---- V: y = rou x - (hdt/h^3)(a - b) on the cells of plane i (planes i-1, i, i+1 of x).
 // With partials, block partials of h^3 sum x.y over the regular cells (their share of p.A p,
 // as in k_sell_v; vol = h^3 on a regular cell)
@@ -453,15 +622,6 @@ __global__ __launch_bounds__(256) void k_sg3_t(int64_t ncell, Levels L, int ci, 
   const Tile t = tile_of(v.ny, v.nz, v.nx, ci, bx, by);
   double dot = 0.0;
   const T alf = rr ? (T)scal[S_ALPHA] : T(0);
-  auto emit = [&](int64_t e, T val) {
-    if (rr) {
-      const T ri = rr[e] - alf * val;
-      rr[e] = ri;
-      dot += (double)(ri * prec[e]) * (double)ri;
-    } else {
-      w[e] = val;
-    }
-  };
   if (t.live) {
     constexpr int PJ = TJ + 1, PK = TK + 1, PS = PJ * PK, R = 2;   // low-side halo (j-1, k-1)
     __shared__ T sy[2][3][PS];
@@ -487,44 +647,81 @@ __global__ __launch_bounds__(256) void k_sg3_t(int64_t ncell, Levels L, int ci, 
         }
       }
     };
+    const T hh = (T)(0.5 * v.h);
+    const int j = t.j0 + t.tj, k = t.k0 + t.tk;
+    const bool mine = j < v.ny && k < v.nz;         // this thread's cell is in the level box
+    const int q = (t.tj + 1) * PK + t.tk + 1;      // cell (i, j, k) in the staged plane
+    const int ny1 = v.ny + 1, nz1 = v.nz + 1;
+    // the x-, y- and z-edge (i, j, k) of this thread's cell
+    auto edge = [&](int i, int d) -> int64_t {
+      return d == 0 ? v.e0 + ((int64_t)i * ny1 + j) * nz1 + k
+                    : d == 1 ? v.e1 + ((int64_t)i * v.ny + j) * nz1 + k : v.e2 + ((int64_t)i * ny1 + j) * v.nz + k;
+    };
+    // per-edge operands of plane i (flag; r and prec in the fused update), loaded one plane ahead
+    struct EdgeOps {
+      T r[3], m[3];
+      uint8_t g[3];
+    };
+    auto efetch = [&](int i, EdgeOps& o) {
+      if (!mine) return;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int64_t e = edge(i, d);
+        o.g[d] = reg[e];
+        if (rr) {
+          o.r[d] = rr[e];
+          o.m[d] = prec[e];
+        }
+      }
+    };
+    auto emit = [&](const EdgeOps& o, int i, int d, T val) {
+      if (!o.g[d]) return;
+      const int64_t e = edge(i, d);
+      if (rr) {
+        const T ri = o.r[d] - alf * val;
+        rr[e] = ri;
+        dot += (double)(ri * o.m[d]) * (double)ri;
+      } else {
+        w[e] = val;
+      }
+    };
     Prefetch<T, 3 * R> f;
+    EdgeOps nxt{}, cur{};
     fetch(t.i0 - 1, f);
     store(t.i0 - 1, f);
     fetch(t.i0, f);
-    const T hh = (T)(0.5 * v.h);
-    const int j = t.j0 + t.tj, k = t.k0 + t.tk;
-    const int q = (t.tj + 1) * PK + t.tk + 1;      // cell (i, j, k) in the staged plane
-    const int ny1 = v.ny + 1, nz1 = v.nz + 1;
+    efetch(t.i0, nxt);
     for (int i = t.i0; i < t.i1; ++i) {
       store(i, f);
+      cur = nxt;
       __syncthreads();
-      if (i + 1 < t.i1) fetch(i + 1, f);          // in flight while plane i is computed
-      if (j < v.ny && k < v.nz) {
+      if (i + 1 < t.i1) {                          // in flight while plane i is computed
+        fetch(i + 1, f);
+        efetch(i + 1, nxt);
+      }
+      if (mine) {
         const int s0 = i & 1, sm = s0 ^ 1;          // planes i and i-1
         auto Y = [&](int s, int d, int dj, int dk) { return sy[s][d][q + dj * PK + dk]; };
         {   // x-edge (i, j, k): -F_y(i,j,k) + F_y(i,j,k-1) + F_z(i,j,k) - F_z(i,j-1,k)
-          const int64_t e = v.e0 + ((int64_t)i * ny1 + j) * nz1 + k;
           const T fy0 = hh * (Y(s0, 1, -1, 0) + Y(s0, 1, 0, 0));
           const T fy1 = hh * (Y(s0, 1, -1, -1) + Y(s0, 1, 0, -1));
           const T fz0 = hh * (Y(s0, 2, 0, -1) + Y(s0, 2, 0, 0));
           const T fz1 = hh * (Y(s0, 2, -1, -1) + Y(s0, 2, -1, 0));
-          if (reg[e]) emit(e, (fy1 - fy0) + (fz0 - fz1));
+          emit(cur, i, 0, (fy1 - fy0) + (fz0 - fz1));
         }
         {   // y-edge (i, j, k): F_x(i,j,k) - F_x(i,j,k-1) - F_z(i,j,k) + F_z(i-1,j,k)
-          const int64_t e = v.e1 + ((int64_t)i * v.ny + j) * nz1 + k;
           const T fx0 = hh * (Y(sm, 0, 0, 0) + Y(s0, 0, 0, 0));
           const T fx1 = hh * (Y(sm, 0, 0, -1) + Y(s0, 0, 0, -1));
           const T fz0 = hh * (Y(s0, 2, 0, -1) + Y(s0, 2, 0, 0));
           const T fz1 = hh * (Y(sm, 2, 0, -1) + Y(sm, 2, 0, 0));
-          if (reg[e]) emit(e, (fx0 - fx1) + (fz1 - fz0));
+          emit(cur, i, 1, (fx0 - fx1) + (fz1 - fz0));
         }
         {   // z-edge (i, j, k): -F_x(i,j,k) + F_x(i,j-1,k) + F_y(i,j,k) - F_y(i-1,j,k)
-          const int64_t e = v.e2 + ((int64_t)i * ny1 + j) * v.nz + k;
           const T fx0 = hh * (Y(sm, 0, 0, 0) + Y(s0, 0, 0, 0));
           const T fx1 = hh * (Y(sm, 0, -1, 0) + Y(s0, 0, -1, 0));
           const T fy0 = hh * (Y(s0, 1, -1, 0) + Y(s0, 1, 0, 0));
           const T fy1 = hh * (Y(sm, 1, -1, 0) + Y(sm, 1, 0, 0));
-          if (reg[e]) emit(e, (fx1 - fx0) + (fy0 - fy1));
+          emit(cur, i, 2, (fx1 - fx0) + (fy0 - fy1));
         }
       }
       __syncthreads();
@@ -679,12 +876,14 @@ inline int slice_red_grid(int64_t nslice) {
   const int g = slice_grid(nslice);
   return g > kRedGrid ? kRedGrid : g;
 }
-// G1 and V passes of A p: x = interp curl p into ws->cell_a, V x into ws->cell_b.  With
-// want_dot, partials of p.A p go to ws->partials [0, *nv) (composite V rows) and
-// [kRedGrid, kRedGrid + *nt) (tiled V pass): p.A p = sum_cells vol x.(V x), so the dot needs
-// no second read of p.  It equals p.w when p is zero on the Dirichlet rows, as every PCG
-// direction is.  Each pass: the tiled stencil writes the regular rows, the composite kernel
-// the others (disjoint rows).  Running the two on separate streams measured no faster.
+// G1 and V passes of A p: y = V x, x = interp curl p, into ws->cell_b.  In float64 x reaches
+// ws->cell_a only on the composite G1 rows and the cells the composite V rows read (fused
+// pass); in float32 the two tiled passes go through ws->cell_a.  With want_dot,
+// partials of p.A p go to ws->partials [0, *nv) (composite V rows) and
+// [kRedGrid, kRedGrid + *nt) (tiled pass): p.A p = sum_cells vol x.(V x), so the dot needs no
+// second read of p.  It equals p.w when p is zero on the Dirichlet rows, as every PCG
+// direction is.  Tiled and composite kernels write disjoint rows; running them on separate
+// streams measured no faster than in order.
 template <typename T>
 void g1_v(const pyib_grid* g, const pyib_work* ws, double rou, double visc, double hdt, const T* p, bool want_dot,
           int* nv, int* nt, hipStream_t st) {
@@ -693,13 +892,21 @@ void g1_v(const pyib_grid* g, const pyib_work* ws, double rou, double visc, doub
   T* cb = static_cast<T*>(ws->cell_b);
   int ci = CI;
   const dim3 gt = tile_grid(L, &ci);
-  k_sg1_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, g->cell_reg_g1, p, ca);
+  // the composite G1 rows first: the fused pass reads their x from HBM
   if (g->g1_nslice > 0)
     k_sell_g1<T><<<slice_grid(g->g1_nslice), kBlock, 0, st>>>(g->g1_nslice, g->ncell, g->g1_row, g->g1_width,
                                                              g->g1_sptr, g->g1_col, (const T*)g->g1_coef, p, ca);
+  double* tpart = want_dot ? ws->partials + kRedGrid : nullptr;
+  if constexpr (sizeof(T) == 8) {
+    // float64: one fused pass (measured 318 us against 167 + 179 us for the two passes at D32)
+    k_sg1v_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, g->cell_g1v, (T)rou, (T)visc, (T)hdt, p, ca, cb, tpart);
+  } else {
+    // float32: the two passes (110 + 99 us against 248 us fused at D32: the fused pass is
+    // latency-bound once its bytes halve)
+    k_sg1_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, g->cell_reg_g1, p, ca);
+    k_sv_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, g->cell_reg, (T)rou, (T)visc, (T)hdt, ca, cb, tpart);
+  }
   const int n1 = slice_red_grid(g->v_nslice);
-  k_sv_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, g->cell_reg, (T)rou, (T)visc, (T)hdt, ca, cb,
-                                   want_dot ? ws->partials + kRedGrid : nullptr);
   if (g->v_nslice > 0)
     k_sell_v<T><<<n1, kBlock, 0, st>>>(g->v_nslice, g->ncell, g->v_row, g->v_width, g->v_sptr, g->v_col,
                                        (const T*)g->v_coef, (T)rou, (T)(hdt * visc), ca, cb, g->v_rw,

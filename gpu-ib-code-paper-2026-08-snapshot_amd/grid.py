@@ -16,7 +16,7 @@ import torch
 
 from . import _native as N
 from .core import dev_stream, ptr
-from .layout import MAX_LEVELS, build_layout, dir_mask_internal, row_volumes, to_sell
+from .layout import MAX_LEVELS, build_layout, dir_mask_internal, g1v_flags, row_volumes, to_sell
 
 
 class HipGrid:
@@ -29,6 +29,7 @@ class HipGrid:
         t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)
         self._cell_reg = t(lay.reg_cell, torch.uint8)
         self._cell_reg_g1 = t(lay.reg_g1, torch.uint8)
+        self._cell_g1v = t(g1v_flags(lay), torch.uint8)
         self._edge_reg = t(lay.reg_edge, torch.uint8)
         self._dir = t(np.ones(lay.nedge, np.uint8), torch.uint8)
         # reference -> internal gather: idx over internal slots; internal -> reference: idx over ref edges
@@ -80,6 +81,7 @@ class HipGrid:
             L.h = lv.h
         g.cell_reg, g.cell_reg_g1 = ptr(self._cell_reg), ptr(self._cell_reg_g1)
         g.edge_reg, g.dir = ptr(self._edge_reg), ptr(self._dir)
+        g.cell_g1v = ptr(self._cell_g1v)
         c, k = self._comp, self._coef[prec]
         for pre, name in (("g1", "g1"), ("v", "vis"), ("g3", "g3")):
             cc = c[name]

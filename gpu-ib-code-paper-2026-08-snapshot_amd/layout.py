@@ -428,6 +428,20 @@ def to_sell(comp, C=SLICE, sigma=8 * SLICE):
     return SellComposite(rows, sw.astype(np.int32), sptr, col, coef)
 
 
+def g1v_flags(lay):
+    """Per internal cell, the flags of the fused G1+V pass: bit 0 = regular G1 row (x =
+    interp curl p formed by the stencil), bit 1 = that x must also be stored, because a
+    composite V row reads it (as its own row or a column), bit 2 = regular V row.  The pass
+    forms x in LDS and stores only the bit-1 cells."""
+    keep = np.zeros(lay.ncell, bool)
+    keep[np.asarray(lay.vis.rows, dtype=np.int64)] = True
+    keep[np.asarray(lay.vis.col, dtype=np.int64)] = True
+    reg = lay.reg_g1.astype(bool)
+    regv = lay.reg_cell.astype(bool)
+    return (reg.astype(np.uint8) | ((reg & keep).astype(np.uint8) << 1)
+            | (regv.astype(np.uint8) << 2)).astype(np.uint8)
+
+
 def row_volumes(lay, geo, rows):
     """Cell volume 1/vol_ci of each internal cell row (0 for -1 padding and for cells with
     vol_ci <= 0).  The composite V kernel weighs each row's x.(V x) by it to form its share

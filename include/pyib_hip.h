@@ -107,6 +107,9 @@ typedef struct pyib_grid {
   const uint8_t *edge_reg;  /* nedge: G3 row computed by the stencil (w must be zero-initialised:
                              * empty slots are never written) */
   const uint8_t *dir;       /* nedge: Dirichlet edge or empty slot (residual forced to 0) */
+  const uint8_t *cell_g1v;  /* ncell, flags of the fused G1+V pass: bit 0 = cell_reg_g1, bit 1 = its
+                             * x = interp(curl p) is stored for the composite V rows (the pass keeps
+                             * the rest in LDS), bit 2 = cell_reg */
   /* composite rows of the other (interface/boundary) rows in sliced ELL: slices of 64
    * rows (one wavefront), entry j of lane l of slice s at sptr[s] + 64*j + l, width[s]
    * entries per lane, row[64*s + l] = internal row id or -1 (padding, col 0, coef 0) */

@@ -11,7 +11,7 @@ import pytest
 import pyib_hip  # noqa: F401
 import pyib_hip.mesh as PM
 from pyib_hip.geometry import Geometry
-from pyib_hip.layout import build_layout, row_volumes, to_internal_edges, to_reference_edges
+from pyib_hip.layout import build_layout, g1v_flags, row_volumes, to_internal_edges, to_reference_edges
 from pyib_hip.tables import build_tables
 from oracle.pyib_ref import core as OCore
 from tests.helpers import rel
@@ -160,3 +160,130 @@ def test_quadratic_form_from_cell_passes(h, boxes):
     assert abs(quad - pw) <= 1e-13 * abs(pw)
     assert abs(quad - ref) <= 1e-13 * abs(ref)
     assert np.all(row_volumes(lay, g, np.array([-1])) == 0.0)
+
+
+def test_g1v_flags_cover_the_composite_v_reads():
+    """The fused G1+V pass stores x only where bit 1 is set; every cell a composite V row
+    reads (its own row and its columns) must then be stored or come from the composite G1
+    rows (not a regular G1 row)."""
+    m = PM.make_nested_mesh(0.0625, PM.nested_box_boxes([1.5, 1.25, 1.25], 3))
+    g = Geometry(m)
+    lay = build_layout(m, g, build_tables(g))
+    f = g1v_flags(lay)
+    assert np.array_equal(f & 1, lay.reg_g1)
+    assert np.array_equal((f >> 2) & 1, lay.reg_cell)
+    assert not np.any((f & 2).astype(bool) & ~(f & 1).astype(bool))
+    read = np.unique(np.concatenate([lay.vis.rows, lay.vis.col]))
+    ok = ((f[read] & 2) != 0) | (lay.reg_g1[read] == 0)
+    assert ok.all()
+    assert 0 < int((f & 2).astype(bool).sum()) < int(lay.reg_g1.sum())
+
+
+# ------------------------------------------------------------- fused G1+V tile loop
+TJ, TK = 8, 32                                   # tile of k_sg1v_t
+RJ, RK = TJ + 2, TK + 2                          # cells with the one-cell halo
+XJ, XK, YJ, YK, ZJ, ZK = RJ + 1, RK + 1, RJ, RK + 1, RJ + 1, RK
+
+
+def _tile(lay, lv, p, flags, xmem, y, j0, k0, i0, i1, rou, visc, hdt):
+    nx, ny, nz, co = lv.nx, lv.ny, lv.nz, lv.cell_off
+    e0, e1, e2 = lv.e_off
+    ny1, nz1 = ny + 1, nz + 1
+    cf, kA, hv = 1.0 / (2 * lv.h * lv.h), lv.h * visc, hdt / lv.h ** 3
+    jb, kb = j0 - 1, k0 - 1
+
+    def edges(d, ip):
+        """Staged edge plane (0 outside the level's edge blocks), as the kernel's fetch."""
+        J, K = {0: (XJ, XK), 1: (YJ, YK), 2: (ZJ, ZK)}[d]
+        q = np.arange(J * K)
+        j, k = jb + q // K, kb + q % K
+        nj, nk, ni = {0: (ny1, nz1, nx), 1: (ny, nz1, nx + 1), 2: (ny1, nz, nx + 1)}[d]
+        ok = (ip >= 0) & (ip < ni) & (j >= 0) & (j < nj) & (k >= 0) & (k < nk)
+        idx = (e0, e1, e2)[d] + (ip * nj + j) * nk + k
+        return np.where(ok, p[np.where(ok, idx, 0)], 0.0)
+
+    def cell_flags(ip):
+        q = np.arange(RJ * RK)
+        j, k = jb + q // RK, kb + q % RK
+        ok = (ip >= 0) & (ip < nx) & (j >= 0) & (j < ny) & (k >= 0) & (k < nz)
+        return np.where(ok, 0x80 | flags[np.where(ok, co + (ip * ny + j) * nz + k, 0)], 0)
+
+    slot = lambda i: (i + 2) & 1              # noqa: E731
+    sy, sz, sc = [None, None], [None, None], [None, None]
+    sy[slot(i0 - 1)], sz[slot(i0 - 1)] = edges(1, i0 - 1), edges(2, i0 - 1)
+    tj, tk = (v.ravel() for v in np.meshgrid(np.arange(TJ), np.arange(TK), indexing="ij"))
+    qo = (tj + 1) * RK + tk + 1
+    xm = np.zeros((3, TJ * TK))
+    q = np.arange(RJ * RK)
+    jj, kk = q // RK, q % RK
+    for ip in range(i0 - 1, i1 + 1):
+        a, b = slot(ip), slot(ip + 1)
+        sx, sy[b], sz[b] = edges(0, ip), edges(1, ip + 1), edges(2, ip + 1)
+        fc = cell_flags(ip)
+        PX = lambda dj, dk: sx[(jj + dj) * XK + kk + dk]      # noqa: E731
+        PY = lambda s, dk: sy[s][jj * YK + kk + dk]           # noqa: E731
+        PZ = lambda s, dj: sz[s][(jj + dj) * ZK + kk]         # noqa: E731
+        ux = (PY(a, 0) - PY(a, 1)) + (PZ(a, 1) - PZ(a, 0)) + ((PY(b, 0) - PY(b, 1)) + (PZ(b, 1) - PZ(b, 0)))
+        uy = (PX(0, 1) - PX(0, 0)) - (PZ(b, 0) - PZ(a, 0)) + ((PX(1, 1) - PX(1, 0)) - (PZ(b, 1) - PZ(a, 1)))
+        uz = (PY(b, 0) - PY(a, 0)) - (PX(1, 0) - PX(0, 0)) + ((PY(b, 1) - PY(a, 1)) - (PX(1, 1) - PX(0, 1)))
+        comp = np.stack([ux * cf, uy * cf, uz * cf])
+        c = co + (ip * ny + (jb + jj)) * nz + (kb + kk)
+        reg, inb = (fc & 1) != 0, (fc & 0x80) != 0
+        mem = np.stack([xmem[3 * np.where(inb, c, 0) + d] for d in range(3)])
+        own = (jj >= 1) & (jj <= TJ) & (kk >= 1) & (kk <= TK) & (ip >= i0) & (ip < i1)
+        wr = reg & ((fc & 2) != 0) & own
+        for d in range(3):
+            xmem[3 * c[wr] + d] = comp[d][wr]
+        sc[a] = np.where(reg, comp, np.where(inb, mem, 0.0))
+        i = ip - 1
+        if i >= i0:
+            j, k = j0 + tj, k0 + tk
+            ok = (j < ny) & (k < nz)
+            cc = co + (i * ny + j) * nz + k
+            rv = ok & ((flags[np.where(ok, cc, 0)] & 4) != 0)
+            s0 = slot(i)
+            for d in range(3):
+                xc = sc[s0][d][qo]
+                aa = kA * (sc[a][d][qo] - xc) + kA * (sc[s0][d][qo + RK] - xc) + kA * (sc[s0][d][qo + 1] - xc)
+                bb = kA * (xc - xm[d]) + kA * (xc - sc[s0][d][qo - RK]) + kA * (xc - sc[s0][d][qo - 1])
+                y[3 * cc[rv] + d] = (rou * xc - hv * (aa - bb))[rv]
+        if ip >= i0:
+            xm = np.stack([sc[slot(ip - 1)][d][qo] for d in range(3)])
+
+
+def emulate_fused_g1v(lay, p, rou, visc, hdt, ci=16):
+    """The fused G1+V tile loop of k_sg1v_t restated in NumPy: same tiles, halo, plane ring,
+    register-kept plane i-1 and store rule.  Returns (y, x as stored in HBM)."""
+    flags = g1v_flags(lay)
+    xmem = np.zeros(3 * lay.ncell)
+    g1 = lay.g1                                  # composite G1 rows run first
+    for r in range(g1.rows.shape[0]):
+        a, b = g1.ptr[r], g1.ptr[r + 1]
+        xmem[3 * g1.rows[r]:3 * g1.rows[r] + 3] = (g1.coef[a:b] * p[g1.col[a:b]][:, None]).sum(axis=0)
+    y = np.zeros(3 * lay.ncell)
+    for lv in lay.levels:
+        for j0 in range(0, lv.ny, TJ):
+            for k0 in range(0, lv.nz, TK):
+                for i0 in range(0, lv.nx, ci):
+                    _tile(lay, lv, p, flags, xmem, y, j0, k0, i0, min(i0 + ci, lv.nx), rou, visc, hdt)
+    return y, xmem
+
+
+@pytest.mark.parametrize("ci", [4, 16])
+def test_fused_g1v_tile_loop_equals_two_passes(ci):
+    """Index math of the fused G1+V pass (halo, plane ring, i-chunks) against the two-pass
+    emulation: y on every regular V cell and x on every stored cell."""
+    m = PM.make_nested_mesh(0.0625, PM.nested_box_boxes([1.5, 1.25, 1.25], 3))
+    g = Geometry(m)
+    lay = build_layout(m, g, build_tables(g))
+    rng = np.random.default_rng(5)
+    p = rng.standard_normal(lay.nedge)
+    p[lay.edge_inv < 0] = 0.0
+    _, vc, v2 = emulate(lay, p, 1.0, 0.45, 0.002, cells=True)
+    y, xmem = emulate_fused_g1v(lay, p, 1.0, 0.45, 0.002, ci=ci)
+    regv = lay.reg_cell.astype(bool)
+    yy = y.reshape(-1, 3).T
+    assert np.abs(yy[:, regv] - v2[:, regv]).max() <= 1e-14 * np.abs(v2).max()
+    keep = (g1v_flags(lay) & 2) != 0
+    xx = xmem.reshape(-1, 3).T
+    assert np.abs(xx[:, keep] - vc[:, keep]).max() <= 1e-14 * np.abs(vc).max()
