@@ -293,7 +293,8 @@ def main(argv=None):
                    "precision": head, "cg_iters_per_step": r["cg_iters_per_step"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(inner_prec),
-                     "kernel": f"one {inner_prec} PCG iteration (A.p + dot, update + dot, direction)",
+                     "kernel": f"one {inner_prec} PCG iteration (G1 and V passes with p.Ap, G3 pass fused with "
+                               "r -= alpha A p and r.prec.r, direction fused with x += alpha p)",
                      "ms_per_iteration": cg["ms_per_iter"],
                      "algorithmic_bytes_per_iteration": bytes_iter},
     }
