@@ -85,6 +85,7 @@ class StepContext:
         self.cg_iterations = 0
         self.adjoint_cg_iterations = 0
         self.projection_iterations = 0
+        self.log = None                      # optional callable(str): one line per adjoint solve
         nb = C.c_int64(0)
         N.call("pyib_ib_sort_temp_bytes", C.c_int64(mo.ref.shape[0] * TAPS), C.byref(nb))
         self._sort_bytes = max(int(nb.value), 1)
@@ -269,9 +270,12 @@ class Helmholtz(torch.autograd.Function):
     def backward(ctx, g):
         K = ctx.K
         g = g.contiguous()
+        p0 = K.projection_iterations
         g_r = project_on_range(K, g, tol=K.tol2)
         g_res, m = K.drv.helm_solve(g_r, torch.zeros_like(g), K.dt, tol1=K.tol1, tol2=K.tol2, cg_max=K.cg_max)
         K.adjoint_cg_iterations += int(m)
+        if K.log:
+            K.log(f"adjoint Helmholtz solve: projection CG {K.projection_iterations - p0}, PCG {int(m)}")
         return None, g_res, g
 
 
@@ -421,6 +425,7 @@ class PeriodForceObjective:
     def __init__(self, rt, steps, checkpoint_every=20, tol1=1e-10, tol2=1e-9, cg_max=None, log=None):
         self.log = log                       # optional callable(str): one line per segment
         self.K = StepContext(rt, tol1=tol1, tol2=tol2, cg_max=cg_max)
+        self.K.log = log
         self.steps = int(steps)
         self.every = max(1, int(checkpoint_every))
         mo = rt.driver.motion

@@ -72,9 +72,10 @@ def test_cfg4_d40_mixed_vs_float64():
 
 
 def test_cfg5_adjoint_d16_against_finite_differences():
-    """K = 6 steps with a checkpoint every 3 (the 200-step period of the full case takes
-    hours of PCG at the tight tolerance a central difference needs)."""
-    steps, every = 6, 3
+    """K = 2 steps with a checkpoint every step, so the backward pass recomputes each segment
+    (the 200-step period of the full case takes hours of PCG at the tight tolerance a
+    central difference needs: about 15,000 PCG iterations per step at D16)."""
+    steps, every = 2, 1
     cfg = full_size_config("cfg5", steps=steps)
     # at 1e-12 / 1e-11 the D16 float64 PCG ends at its round-off floor, where the reference's
     # strict test can miss by a few per cent (true residual vs the recursion's); J is then
