@@ -329,16 +329,26 @@ __global__ __launch_bounds__(256) void k_sg1_t(int64_t ncell, Levels L, int ci, 
     const bool in = t.i0 <= v.nx && j < ny1 && k < v.nz;
     sz[t.i0 & 1][q] = in ? p[v.e2 + ((int64_t)t.i0 * ny1 + j) * v.nz + k] : T(0);
   }
-  Prefetch<T, R> fx, fy, fz;
-  fetch(t.i0, fx, fy, fz);
-  const T cf = (T)(1.0 / (2.0 * v.h * v.h));
+  // two planes in flight: buffer A serves the even steps from i0, B the odd ones; each step
+  // stores its plane, then reloads its buffer two planes ahead (with the cell's flag)
+  struct Pre {
+    Prefetch<T, R> x, y, z;
+    uint8_t g;
+  };
   const int j = t.j0 + t.tj, k = t.k0 + t.tk;
-  for (int i = t.i0; i < t.i1; ++i) {
+  const bool mine = j < v.ny && k < v.nz;
+  auto fetch_all = [&](int i, Pre& f) {
+    fetch(i, f.x, f.y, f.z);
+    f.g = (mine && i < v.nx) ? reg[v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k] : (uint8_t)0;
+  };
+  const T cf = (T)(1.0 / (2.0 * v.h * v.h));
+  auto step = [&](int i, Pre& f) {
     const int a = i & 1, b = a ^ 1;             // slot of plane i, plane i+1
-    store(i, fx, fy, fz);
+    store(i, f.x, f.y, f.z);
+    const uint8_t g = f.g;
     __syncthreads();
-    if (i + 1 < t.i1) fetch(i + 1, fx, fy, fz); // in flight while plane i is computed
-    if (j < v.ny && k < v.nz) {
+    if (i + 2 < t.i1) fetch_all(i + 2, f);      // in flight while planes i and i+1 are computed
+    if (g) {
       auto PX = [&](int dj, int dk) { return sx[a][(t.tj + dj) * XK + t.tk + dk]; };
       auto PY = [&](int s, int dk) { return sy[s][t.tj * YK + t.tk + dk]; };
       auto PZ = [&](int s, int dj) { return sz[s][(t.tj + dj) * ZK + t.tk]; };
@@ -349,13 +359,18 @@ __global__ __launch_bounds__(256) void k_sg1_t(int64_t ncell, Levels L, int ci, 
       const T uz0 = (PY(b, 0) - PY(a, 0)) - (PX(1, 0) - PX(0, 0));
       const T uz1 = (PY(b, 1) - PY(a, 1)) - (PX(1, 1) - PX(0, 1));
       const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
-      if (reg[c]) {
-        vc[3 * c] = (ux0 + ux1) * cf;
-        vc[3 * c + 1] = (uy0 + uy1) * cf;
-        vc[3 * c + 2] = (uz0 + uz1) * cf;
-      }
+      vc[3 * c] = (ux0 + ux1) * cf;
+      vc[3 * c + 1] = (uy0 + uy1) * cf;
+      vc[3 * c + 2] = (uz0 + uz1) * cf;
     }
     __syncthreads();
+  };
+  Pre fa, fb;
+  fetch_all(t.i0, fa);
+  if (t.i0 + 1 < t.i1) fetch_all(t.i0 + 1, fb);
+  for (int i = t.i0; i < t.i1; i += 2) {
+    step(i, fa);
+    if (i + 1 < t.i1) step(i + 1, fb);
   }
 }
 
@@ -566,22 +581,29 @@ __global__ __launch_bounds__(256) void k_sv_t(int64_t ncell, Levels L, int ci, c
         }
       }
     };
-    Prefetch<T, 3 * R> f;
-    fetch(t.i0 - 1, f);
-    store(t.i0 - 1, f);
-    fetch(t.i0, f);
-    store(t.i0, f);
-    fetch(t.i0 + 1, f);
     const T kA = (T)v.h * visc;
     const T hv = hdt * (T)(1.0 / (v.h * v.h * v.h));
     const int j = t.j0 + t.tj, k = t.k0 + t.tk;
+    const bool mine = j < v.ny && k < v.nz;
     const int q = (t.tj + 1) * PK + t.tk + 1;
-    for (int i = t.i0; i < t.i1; ++i) {
-      store(i + 1, f);
+    // two planes in flight: a buffer holds plane i+1 of x and the flag of this thread's cell
+    // on plane i; buffer A serves the even steps from i0, B the odd ones
+    struct Pre {
+      Prefetch<T, 3 * R> x;
+      uint8_t g;
+    };
+    auto fetch_all = [&](int i, Pre& f) {
+      fetch(i, f.x);
+      f.g = (mine && i - 1 >= 0 && i - 1 < v.nx) ? reg[v.cell_off + ((int64_t)(i - 1) * v.ny + j) * v.nz + k]
+                                                  : (uint8_t)0;
+    };
+    auto step = [&](int i, Pre& f) {
+      store(i + 1, f.x);
+      const uint8_t g = f.g;
       __syncthreads();
-      if (i + 1 < t.i1) fetch(i + 2, f);          // in flight while plane i is computed
-      const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
-      if (j < v.ny && k < v.nz && reg[c]) {
+      if (i + 3 <= t.i1) fetch_all(i + 3, f);   // in flight while planes i and i+1 are computed
+      if (g) {
+        const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
         const int sm = (i + 2) % 3, s0 = i % 3, sp = (i + 1) % 3;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -596,6 +618,20 @@ __global__ __launch_bounds__(256) void k_sv_t(int64_t ncell, Levels L, int ci, c
         }
       }
       __syncthreads();
+    };
+    {
+      Prefetch<T, 3 * R> f;
+      fetch(t.i0 - 1, f);
+      store(t.i0 - 1, f);
+      fetch(t.i0, f);
+      store(t.i0, f);
+    }
+    Pre fa, fb;
+    fetch_all(t.i0 + 1, fa);
+    if (t.i0 + 2 <= t.i1) fetch_all(t.i0 + 2, fb);
+    for (int i = t.i0; i < t.i1; i += 2) {
+      step(i, fa);
+      if (i + 1 < t.i1) step(i + 1, fb);
     }
   }
   if (partials) {
