@@ -642,7 +642,8 @@ __global__ __launch_bounds__(256) void k_sv_t(int64_t ncell, Levels L, int ci, c
 }
 
 // ---- G3: w on the regular x-, y- and z-edges of plane i (planes i-1 and i of y); the
-// composite pass writes the irregular rows, empty slots are never written.
+// composite pass writes the irregular rows; empty slots (and Dirichlet rows when the caller
+// leaves them out of the composite) are never written.
 // With rr non-NULL the PCG residual update is fused in: w is not stored, r -= alpha w on the
 // same rows (alpha = scal[S_ALPHA], already final since p.A p comes from the V pass) and
 // block partials of r.prec.r go to partials[block]

@@ -16,7 +16,8 @@ import torch
 
 from . import _native as N
 from .core import dev_stream, ptr
-from .layout import MAX_LEVELS, build_layout, dir_mask_internal, g1v_flags, row_volumes, to_sell
+from .layout import (MAX_LEVELS, build_layout, composite_rows, dir_mask_internal, g1v_flags, row_volumes,
+                     to_sell)
 
 
 class HipGrid:
@@ -37,19 +38,23 @@ class HipGrid:
         self._to_int_sig = t(lay.sigma_int, torch.int8)
         self._to_ref_idx = t(lay.edge_map, torch.int64)
         self._to_ref_sig = t(lay.edge_sigma, torch.int8)
+        self._t = t
         self._comp = {}
         for name in ("g1", "vis", "g3"):
-            c = to_sell(getattr(lay, name))
-            self._comp[name] = {"rows": t(c.rows, torch.int32), "width": t(c.width, torch.int32),
-                                "sptr": t(c.sptr, torch.int64), "col": t(c.col, torch.int32),
-                                "n": int(c.width.shape[0]), "coef64": c.coef}
+            self._comp[name] = self._sell(getattr(lay, name))
             if name == "vis":
-                self._comp[name]["rw"] = t(row_volumes(lay, core.geo, c.rows), torch.float64)
+                self._comp[name]["rw"] = t(row_volumes(lay, core.geo, self._comp[name]["rows_host"]), torch.float64)
         self._coef = {}
         self._struct = {}
         self.work = {}
         for prec in precisions:
             self.add_precision(prec)
+
+    def _sell(self, comp):
+        t = self._t
+        c = to_sell(comp)
+        return {"rows": t(c.rows, torch.int32), "width": t(c.width, torch.int32), "sptr": t(c.sptr, torch.int64),
+                "col": t(c.col, torch.int32), "n": int(c.width.shape[0]), "coef64": c.coef, "rows_host": c.rows}
 
     def add_precision(self, prec):
         if prec in self._struct:
@@ -95,7 +100,17 @@ class HipGrid:
         return g
 
     def set_dir_mask(self, dir_mask_ref):
-        self._dir.copy_(torch.from_numpy(dir_mask_internal(self.lay, dir_mask_ref)))
+        """Dirichlet rows (and empty slots) are never written by the G3 passes: their output is
+        zero (w = 0, r unchanged at 0), so the G3 composite keeps only the other rows."""
+        m = dir_mask_internal(self.lay, dir_mask_ref)
+        self._dir.copy_(torch.from_numpy(m))
+        g3 = self.lay.g3
+        self._comp["g3"] = self._sell(composite_rows(g3, np.flatnonzero(m[np.asarray(g3.rows)] == 0)))
+        for prec in self._struct:
+            dt = torch.float64 if prec == "float64" else torch.float32
+            self._coef[prec]["g3"] = torch.from_numpy(np.ascontiguousarray(self._comp["g3"]["coef64"])).to(
+                self.device, dt)
+            self._struct[prec] = self._make(prec)
 
     def struct(self, prec="float64"):
         return C.byref(self._struct[prec])

@@ -428,6 +428,18 @@ def to_sell(comp, C=SLICE, sigma=8 * SLICE):
     return SellComposite(rows, sw.astype(np.int32), sptr, col, coef)
 
 
+def composite_rows(comp, sel):
+    """The composite made of rows ``sel`` (indices into comp.rows) of ``comp``, each row's
+    entries in their original order."""
+    sel = np.asarray(sel, dtype=np.int64)
+    ptr = np.asarray(comp.ptr, dtype=np.int64)
+    width = ptr[sel + 1] - ptr[sel]
+    new_ptr = np.zeros(sel.shape[0] + 1, np.int64)
+    np.cumsum(width, out=new_ptr[1:])
+    ent = np.repeat(ptr[sel], width) + (np.arange(int(new_ptr[-1])) - np.repeat(new_ptr[:-1], width))
+    return Composite(np.asarray(comp.rows)[sel], new_ptr, np.asarray(comp.col)[ent], np.asarray(comp.coef)[ent])
+
+
 def g1v_flags(lay):
     """Per internal cell, the flags of the fused G1+V pass: bit 0 = regular G1 row (x =
     interp curl p formed by the stencil), bit 1 = that x must also be stored, because a

@@ -105,7 +105,8 @@ typedef struct pyib_grid {
   const uint8_t *cell_reg;  /* ncell: V row computed by the stencil */
   const uint8_t *cell_reg_g1; /* ncell: G1 row computed by the stencil */
   const uint8_t *edge_reg;  /* nedge: G3 row computed by the stencil (w must be zero-initialised:
-                             * empty slots are never written) */
+                             * empty slots, and Dirichlet rows left out of the G3 composite, are
+                             * never written) */
   const uint8_t *dir;       /* nedge: Dirichlet edge or empty slot (residual forced to 0) */
   const uint8_t *cell_g1v;  /* ncell, flags of the fused G1+V pass: bit 0 = cell_reg_g1, bit 1 = its
                              * x = interp(curl p) is stored for the composite V rows (the pass keeps
@@ -173,7 +174,8 @@ int pyib_extrapolate(int64_t n, const double *s, const double *s_old, double dt,
 
 /* ---- the same CG on the structured layout (ib_driver_np.py:311-410) ----
  * Workspace cell_a/cell_b hold 3*ncell (internal) values; partials >= 4*PYIB_MAX_PARTIALS.
- * flow_mat: w = A p (0 on dir).  If dot_out is non-NULL, *dot_out (device) receives
+ * flow_mat: w = A p (0 on dir; a caller that drops the Dirichlet rows from the G3 composite
+ * zero-initialises w).  If dot_out is non-NULL, *dot_out (device) receives
  * p.A p = sum_cells vol x.(V x), x = interp(curl p), in double (deterministic): p.w when
  * p is zero on the dir rows. */
 int pyib_grid_flow_mat(const pyib_grid *g, const pyib_work *ws, double rou, double visc, double hdt,

@@ -11,7 +11,8 @@ import pytest
 import pyib_hip  # noqa: F401
 import pyib_hip.mesh as PM
 from pyib_hip.geometry import Geometry
-from pyib_hip.layout import build_layout, g1v_flags, row_volumes, to_internal_edges, to_reference_edges
+from pyib_hip.layout import (build_layout, composite_rows, g1v_flags, row_volumes, to_internal_edges,
+                             to_reference_edges)
 from pyib_hip.tables import build_tables
 from oracle.pyib_ref import core as OCore
 from tests.helpers import rel
@@ -287,3 +288,19 @@ def test_fused_g1v_tile_loop_equals_two_passes(ci):
     keep = (g1v_flags(lay) & 2) != 0
     xx = xmem.reshape(-1, 3).T
     assert np.abs(xx[:, keep] - vc[:, keep]).max() <= 1e-14 * np.abs(vc).max()
+
+
+def test_composite_rows_keeps_each_selected_row():
+    """Dropping rows (the Dirichlet rows of the G3 composite) keeps every other row's entries."""
+    m = PM.make_nested_mesh(0.125, PM.concentric_boxes())
+    g = Geometry(m)
+    lay = build_layout(m, g, build_tables(g))
+    c = lay.g3
+    sel = np.flatnonzero(np.arange(c.rows.shape[0]) % 3 != 1)
+    sub = composite_rows(c, sel)
+    assert np.array_equal(sub.rows, c.rows[sel])
+    for i, r in enumerate(sel[:200]):
+        a, b = c.ptr[r], c.ptr[r + 1]
+        assert np.array_equal(sub.col[sub.ptr[i]:sub.ptr[i + 1]], c.col[a:b])
+        assert np.array_equal(sub.coef[sub.ptr[i]:sub.ptr[i + 1]], c.coef[a:b])
+    assert int(sub.ptr[-1]) == int(sum(c.ptr[r + 1] - c.ptr[r] for r in sel))
