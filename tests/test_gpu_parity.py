@@ -151,6 +151,37 @@ def test_grid_flow_mat(h, boxes):
     assert abs(float(dot32.item()) - pap) <= 1e-5 * abs(pap)
 
 
+def test_grid_flow_mat_without_dirichlet_composite_rows():
+    """With a cavity's Dirichlet mask the G3 composite drops its Dirichlet rows: w is still the
+    oracle's flow_mat on every other edge, stays exactly 0 on the Dirichlet edges, and the
+    PCG's p.A p (p zero there, as every direction is) is unchanged."""
+    from pyib_hip.grid import HipGrid
+    m = PM.make_nested_mesh(0.0625, PM.nested_box_boxes([1.5, 1.25, 1.25], 3))
+    hc, oc = HipCore(m), OCore.Core(m)
+    g = HipGrid(hc, precisions=("float64", "float32"))
+    dm = np.asarray(m.bc_edge) > 0
+    g.set_dir_mask(np.zeros(oc.nedge, bool))
+    n_all = g._comp["g3"]["n"]
+    g.set_dir_mask(dm)
+    assert g._comp["g3"]["n"] < n_all
+    rng = np.random.default_rng(9)
+    p = rng.standard_normal(oc.nedge)
+    p[dm] = 0.0
+    kw = dict(visc=0.45, rou=1.0, dt=0.0043633)
+    ref = oc.flow_mat(p.copy(), alpd=0.5, **kw)
+    ref[dm] = 0.0
+    w_int, dot = g.flow_mat(g.to_internal(T(p)), kw["rou"], kw["visc"], kw["dt"], with_dot=True)
+    w = H(g.to_reference(w_int))
+    assert rel(w, ref) < 1e-13
+    assert np.all(w[dm] == 0.0)
+    pap = float(p @ ref)
+    assert abs(float(dot.item()) - pap) <= 1e-12 * abs(pap)
+    w32 = H(g.to_reference(g.flow_mat(g.to_internal(T(p), dtype=torch.float32), kw["rou"], kw["visc"], kw["dt"]),
+                           dtype=torch.float64))
+    assert rel(w32, ref) < 1e-5
+    assert np.all(w32[dm] == 0.0)
+
+
 def _run_hip(cfg, steps):
     rt = HC.build_runtime(cfg)
     s, s_old, vc = HC.initial_state(rt)
