@@ -88,7 +88,8 @@ typedef struct pyib_cg {
 
 /* Structured per-level layout of the elliptic operator (one box per 2:1 level).
  * Edge blocks per level: x (nx, ny+1, nz+1), y (nx+1, ny, nz+1), z (nx+1, ny+1, nz),
- * +axis oriented; cell block (nx, ny, nz); cell vectors component-major. */
+ * +axis oriented; cell block (nx, ny, nz); cell vectors interleaved (x, y, z of a cell
+ * adjacent: element 3*c + d). */
 #define PYIB_MAX_LEVELS 8
 typedef struct pyib_level {
   int32_t nx, ny, nz, reserved;

@@ -2,6 +2,10 @@
 import ctypes
 import os
 import re
+import shutil
+import subprocess
+
+import pytest
 
 import pyib_hip  # noqa: F401
 from pyib_hip import _native as N
@@ -44,3 +48,28 @@ def test_load_fails_loudly_when_missing(tmp_path):
         assert raised
     finally:
         N._lib = saved
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+def test_struct_offsets_match_the_c_compiler(tmp_path):
+    """Every ctypes mirror of a C-ABI struct has the C compiler's size and field offsets."""
+    structs = {"pyib_mesh": N.Mesh, "pyib_work": N.Work, "pyib_cg": N.CG, "pyib_level": N.Level,
+               "pyib_grid": N.Grid}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pyib_hip.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for f in cls._fields_:
+            lines.append(f'  printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "offsets.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "offsets"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        struct, field, value = line.split()
+        got[(struct, field)] = int(value)
+    for cname, cls in structs.items():
+        assert got[(cname, "sizeof")] == ctypes.sizeof(cls), cname
+        for f in cls._fields_:
+            assert got[(cname, f[0])] == getattr(cls, f[0]).offset, (cname, f[0])
