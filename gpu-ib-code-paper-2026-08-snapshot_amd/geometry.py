@@ -5,81 +5,55 @@ Computes the same quantities as the reference ``IBCore.__init__``
 (4 links per face + ``link_fe_insert``), edge midpoints/tangents, face and cell
 centres, the per-link circulation sign, face area vectors, the 2:1
 "Modify_geom" centre corrections, ``rf/rf1/rf2``, ``vol_ci`` and ``AoL``.
-Reductions are ``np.bincount`` in link order, which sums in the same order as
-the reference's ``np.add.at``.  Everything here runs once per mesh; the device
-kernels read the derived coefficient tables built in ``tables.py``.
+The work runs in C++ (``pyib_geometry`` in ``csrc/geometry.cpp``). Its sums keep the
+reference's link order, so every value is bitwise the NumPy statement's. Everything here
+runs once per mesh; the device kernels read the derived coefficient tables built in
+``tables.py``.
 """
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 
+from . import _native as N
 
-def _bc3(idx, vals, n):
-    return np.stack([np.bincount(idx, vals[:, d], minlength=n) for d in range(3)], axis=1)
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
 
 
 class Geometry:
     def __init__(self, mesh, ND=3):
+        if ND != 3:
+            raise ValueError("the geometry is three-dimensional")
         self.ND = ND
-        nodes = mesh.nodes
-        E2N = mesh.edges_to_nodes
-        self.F2C = F2C = mesh.faces_to_cells
-        F2E = mesh.faces_to_edges
+        nodes = _c(mesh.nodes, np.float64)
+        e2n = _c(mesh.edges_to_nodes, np.int64)
+        self.F2C = f2c = _c(mesh.faces_to_cells, np.int64)
+        f2e = _c(mesh.faces_to_edges, np.int64)
+        ins = _c(np.asarray(mesh.link_fe_insert).reshape(-1, 2), np.int64)
         self.ncell = ncell = int(mesh.cells.shape[0])
-        self.nface = nface = int(F2C.shape[0])
-        self.nedge = int(E2N.shape[0])
-        self.m1 = m1 = F2C[:, 1] >= 0
-        ins = np.asarray(mesh.link_fe_insert)
-        self.lf_face = lf_face = np.concatenate([np.repeat(np.arange(nface), 4), ins[:, 0]])
-        self.lf_edge = lf_edge = np.concatenate([F2E.ravel(), ins[:, 1]])
-        self.epos = epos = 0.5 * (nodes[E2N[:, 0]] + nodes[E2N[:, 1]])
-        self.te = te = nodes[E2N[:, 1]] - nodes[E2N[:, 0]]
-        fcnt = np.bincount(lf_face, minlength=nface).astype(np.float64)
-        fpos = _bc3(lf_face, epos[lf_edge], nface) / fcnt[:, None]
-        c1, c2 = F2C[:, 0], F2C[m1, 1]
-
-        cboth = np.concatenate([c1, c2])        # c1 owners first, then c2: the reference's add order
-        ccnt = np.bincount(cboth, minlength=ncell).astype(np.float64)
-
-        def cells_avg(fp):
-            return _bc3(cboth, np.concatenate([fp, fp[m1]]), ncell) / ccnt[:, None]
-
-        cpos = cells_avg(fpos)
-        rf = np.empty((nface, 3))
-        rf[m1] = cpos[F2C[m1, 1]] - cpos[F2C[m1, 0]]
-        rf[~m1] = fpos[~m1] - cpos[F2C[~m1, 0]]
-        avec = np.cross(epos[lf_edge] - fpos[lf_face], te[lf_edge])
-        sgn = np.sign((avec * rf[lf_face]).sum(axis=1))
-        sgn[sgn == 0] = 1.0
-        self.fe_sign_link = sgn
-        nf = _bc3(lf_face, avec * sgn[:, None], nface) * (1.0 / (ND - 1))
-        del avec
-        self.nf = nf
-        self.Af = Af = np.linalg.norm(nf, axis=1)
-        # 2:1 interface centre corrections (ib_core_np.py:123-147)
-        tol = 1e-6 * Af
-        rfe = epos[lf_edge] - fpos[lf_face]
-        hitf = np.abs((rfe * rfe).sum(1) - 0.36 * Af[lf_face]) < tol[lf_face]
-        np.add.at(fpos, lf_face[hitf], rfe[hitf] / 6.0)
-        del rfe
-        cpos = cells_avg(fpos)
-        rf1 = fpos - cpos[F2C[:, 0]]
-        rf2 = np.zeros((nface, 3))
-        rf2[m1] = fpos[m1] - cpos[F2C[m1, 1]]
-        a49 = (4.0 / 9.0) * Af
-        h1 = np.abs((rf1 * rf1).sum(1) - a49) < tol
-        np.add.at(cpos, F2C[h1, 0], rf1[h1] / 4.0)
-        h2 = (~h1) & m1 & (np.abs((rf2 * rf2).sum(1) - a49) < tol)
-        np.add.at(cpos, F2C[h2, 1], rf2[h2] / 4.0)
-        self.fpos, self.cpos = fpos, cpos
-        self.rf1 = rf1 = fpos - cpos[F2C[:, 0]]
-        self.rf2 = rf2 = np.zeros((nface, 3))
-        rf2[m1] = fpos[m1] - cpos[F2C[m1, 1]]
-        self.rf = rf = rf1 - rf2
-        vraw = np.bincount(cboth, np.concatenate([(nf * rf1).sum(axis=1), -(nf[m1] * rf2[m1]).sum(axis=1)]),
-                           minlength=ncell)
-        self.vol_ci = np.where(np.abs(vraw) > 1e-14, ND / vraw, 0.0)
-        self.AoL = (rf * nf).sum(axis=1) / (rf * rf).sum(axis=1)
+        self.nface = nface = int(f2c.shape[0])
+        self.nedge = nedge = int(e2n.shape[0])
+        self.m1 = f2c[:, 1] >= 0
+        nlink = 4 * nface + ins.shape[0]
+        self.lf_face = np.empty(nlink, np.int64)
+        self.lf_edge = np.empty(nlink, np.int64)
+        self.epos, self.te = np.empty((nedge, 3)), np.empty((nedge, 3))
+        self.fpos, self.cpos = np.empty((nface, 3)), np.empty((ncell, 3))
+        self.fe_sign_link = np.empty(nlink)
+        self.nf, self.Af = np.empty((nface, 3)), np.empty(nface)
+        self.rf1, self.rf2, self.rf = np.empty((nface, 3)), np.empty((nface, 3)), np.empty((nface, 3))
+        self.vol_ci, self.AoL = np.empty(ncell), np.empty(nface)
+        N.call("pyib_geometry", C.c_int64(nodes.shape[0]), _p(nodes), C.c_int64(nedge), _p(e2n), C.c_int64(nface),
+               _p(f2c), _p(f2e), C.c_int64(ins.shape[0]), _p(ins), C.c_int64(ncell), _p(self.lf_face),
+               _p(self.lf_edge), _p(self.epos), _p(self.te), _p(self.fpos), _p(self.cpos), _p(self.fe_sign_link),
+               _p(self.nf), _p(self.Af), _p(self.rf1), _p(self.rf2), _p(self.rf), _p(self.vol_ci), _p(self.AoL))
 
 
 def diag_preconditioner(geo, dir_mask):

@@ -173,6 +173,19 @@ int pyib_cg_true_residual(const pyib_mesh *m, const pyib_work *ws, const pyib_cg
 int pyib_extrapolate(int64_t n, const double *s, const double *s_old, double dt, double dt_old,
                      double *out, void *stream);
 
+/* ---- mesh geometry on the host (setup; IBCore.__init__, ib_core_np.py:68-166) ----
+ * Host pointers, float64 / int64, row-major.  Inputs: nodes (nnode,3), e2n (nedge,2),
+ * f2c (nface,2; -1 = no neighbour), f2e (nface,4), ins (nins,2) = link_fe_insert (face, edge).
+ * Outputs (preallocated): lf_face, lf_edge (nlink = 4 nface + nins), epos, te (nedge,3),
+ * fpos (nface,3), cpos (ncell,3), sgn (nlink), nf (nface,3), af (nface), rf1, rf2, rf
+ * (nface,3), vol_ci (ncell), aol (nface).  Sums follow the reference's link order, so the
+ * values are bitwise those of its NumPy code. */
+int pyib_geometry(int64_t nnode, const double *nodes, int64_t nedge, const int64_t *e2n, int64_t nface,
+                  const int64_t *f2c, const int64_t *f2e, int64_t nins, const int64_t *ins, int64_t ncell,
+                  int64_t *lf_face, int64_t *lf_edge, double *epos, double *te, double *fpos, double *cpos,
+                  double *sgn, double *nf, double *af, double *rf1, double *rf2, double *rf, double *vol_ci,
+                  double *aol);
+
 /* ---- the same CG on the structured layout (ib_driver_np.py:311-410) ----
  * Workspace cell_a/cell_b hold 3*ncell (internal) values; partials >= 4*PYIB_MAX_PARTIALS.
  * flow_mat: w = A p (0 on dir; a caller that drops the Dirichlet rows from the G3 composite
