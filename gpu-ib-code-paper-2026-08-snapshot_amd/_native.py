@@ -118,6 +118,8 @@ _SIGS = {
     "pyib_fluid_load_adjoint": (C.c_int, [C.c_int64, vp, vp, C.c_double, C.c_double, C.c_double, vp, vp]),
     "pyib_geometry": (C.c_int, [C.c_int64, vp, C.c_int64, vp, C.c_int64, vp, vp, C.c_int64, vp, C.c_int64,
                                 vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "pyib_tables": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_int64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                              vp, vp]),
     "pyib_interior_momentum_adjoint": (C.c_int, [C.c_int64, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                                  C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                                  C.c_double, vp, vp, vp, vp]),

@@ -164,3 +164,60 @@ extern "C" int pyib_geometry(int64_t nnode, const double* nodes, int64_t nedge, 
   for (int64_t f = 0; f < nface; ++f) aol[f] = dot3(rf + 3 * f, nf + 3 * f) / dot3(rf + 3 * f, rf + 3 * f);
   return 0;
 }
+
+// Gather tables of the incidence operators (the CSR the gather kernels read), by stable
+// counting sorts: the curl table lists each face's links in link order, the rot table each
+// edge's links in link order, and the cell-face table each cell's faces as first owner, then
+// as second owner, each in face order (the order of the reference's bincount(c1) - bincount(c2)).
+// Signs are folded into the index: v >= 0 -> +x[v], v < 0 -> -x[-v-1].  Boundary face k (k-th
+// in face order) has the ghost index ncell + k as its opposite owner.
+extern "C" int pyib_tables(int64_t nface, int64_t nedge, int64_t ncell, int64_t nlink, const int64_t* lf_face,
+                           const int64_t* lf_edge, const double* sgn, const int64_t* f2c, int32_t* curl_ptr,
+                           int32_t* curl_idx, int32_t* rot_ptr, int32_t* rot_idx, int32_t* cf_ptr, int32_t* cf_mid,
+                           int32_t* cf_face, int32_t* cf_other) {
+  if (nface >= INT32_MAX || nedge >= INT32_MAX || ncell >= INT32_MAX || nlink >= INT32_MAX) return 1;
+  auto enc = [](int64_t v, double s) { return (int32_t)(s > 0.0 ? v : -v - 1); };
+  // curl: bucket the links by face
+  std::vector<int64_t> pos(nface + 1, 0);
+  for (int64_t l = 0; l < nlink; ++l) ++pos[lf_face[l] + 1];
+  for (int64_t f = 0; f < nface; ++f) pos[f + 1] += pos[f];
+  for (int64_t f = 0; f <= nface; ++f) curl_ptr[f] = (int32_t)pos[f];
+  for (int64_t l = 0; l < nlink; ++l) curl_idx[pos[lf_face[l]]++] = enc(lf_edge[l], sgn[l]);
+  // rot: bucket the links by edge
+  pos.assign(nedge + 1, 0);
+  for (int64_t l = 0; l < nlink; ++l) ++pos[lf_edge[l] + 1];
+  for (int64_t e = 0; e < nedge; ++e) pos[e + 1] += pos[e];
+  for (int64_t e = 0; e <= nedge; ++e) rot_ptr[e] = (int32_t)pos[e];
+  for (int64_t l = 0; l < nlink; ++l) rot_idx[pos[lf_edge[l]]++] = enc(lf_face[l], sgn[l]);
+  // cell <- face: first-owner faces, then second-owner faces, each in face order
+  std::vector<int64_t> n0(ncell, 0), n1(ncell, 0);
+  for (int64_t f = 0; f < nface; ++f) {
+    ++n0[f2c[2 * f]];
+    if (f2c[2 * f + 1] >= 0) ++n1[f2c[2 * f + 1]];
+  }
+  std::vector<int64_t> p0(ncell), p1(ncell);
+  int64_t acc = 0;
+  for (int64_t c = 0; c < ncell; ++c) {
+    cf_ptr[c] = (int32_t)acc;
+    cf_mid[c] = (int32_t)(acc + n0[c]);
+    p0[c] = acc;
+    p1[c] = acc + n0[c];
+    acc += n0[c] + n1[c];
+  }
+  cf_ptr[ncell] = (int32_t)acc;
+  int64_t ghost = ncell;
+  for (int64_t f = 0; f < nface; ++f) {
+    const int64_t c1 = f2c[2 * f], c2 = f2c[2 * f + 1];
+    const int64_t i = p0[c1]++;
+    cf_face[i] = (int32_t)f;
+    cf_other[i] = (int32_t)(c2 >= 0 ? c2 : ghost++);
+  }
+  for (int64_t f = 0; f < nface; ++f) {
+    const int64_t c2 = f2c[2 * f + 1];
+    if (c2 < 0) continue;
+    const int64_t i = p1[c2]++;
+    cf_face[i] = (int32_t)f;
+    cf_other[i] = (int32_t)f2c[2 * f];
+  }
+  return 0;
+}

@@ -1,4 +1,4 @@
-"""Gather tables of the incidence operators (host setup, NumPy).
+"""Gather tables of the incidence operators (host setup; built in C++ by ``pyib_tables``).
 
 Each device kernel is a gather over one of these tables, so every output
 element is written by exactly one thread (no atomics) and visits its inputs in
@@ -15,24 +15,12 @@ the same order as the reference's ``bincount`` over the flat link list
 """
 from __future__ import annotations
 
+import ctypes as C
 from dataclasses import dataclass
 
 import numpy as np
 
-
-def signed(idx, sign):
-    """Encode a +-1 sign into an index: v >= 0 -> +x[v], v < 0 -> -x[-v-1]."""
-    idx = np.asarray(idx, np.int64)
-    return np.where(np.asarray(sign) > 0, idx, -idx - 1).astype(np.int32)
-
-
-def _ptr(rows, n):
-    counts = np.bincount(rows, minlength=n)
-    ptr = np.zeros(n + 1, np.int64)
-    np.cumsum(counts, out=ptr[1:])
-    if ptr[-1] >= 2 ** 31:
-        raise ValueError("table too large for int32 offsets")
-    return ptr.astype(np.int32)
+from . import _native as N
 
 
 @dataclass
@@ -54,35 +42,29 @@ class Tables:
 
 
 def build_tables(geo) -> Tables:
+    """The tables by stable counting sorts in C++ (``pyib_tables``, host only)."""
     nface, nedge, ncell = geo.nface, geo.nedge, geo.ncell
     if max(nface, nedge, ncell) >= 2 ** 31 - 1:
         raise ValueError("mesh too large for int32 indices")
-    lf_face, lf_edge, sgn = geo.lf_face, geo.lf_edge, geo.fe_sign_link
-    o = np.argsort(lf_face, kind="stable")
-    curl_ptr = _ptr(lf_face, nface)
-    curl_idx = signed(lf_edge[o], sgn[o])
-    o = np.argsort(lf_edge, kind="stable")
-    rot_ptr = _ptr(lf_edge, nedge)
-    rot_idx = signed(lf_face[o], sgn[o])
-    del o
-    F2C = geo.F2C
-    m1 = geo.m1
-    bf = np.flatnonzero(~m1)
-    ghost = np.full(nface, -1, np.int64)
-    ghost[bf] = ncell + np.arange(bf.shape[0])
-    f = np.arange(nface)
-    fi = f[m1]
-    cell = np.concatenate([F2C[:, 0], F2C[m1, 1]])
-    part = np.concatenate([np.zeros(nface, np.int8), np.ones(fi.shape[0], np.int8)])
-    face = np.concatenate([f, fi])
-    other = np.concatenate([np.where(m1, F2C[:, 1], ghost), F2C[m1, 0]])
-    order = np.lexsort((face, part, cell))
-    cell, part, face, other = cell[order], part[order], face[order], other[order]
-    cf_ptr = _ptr(cell, ncell)
-    first = np.bincount(cell[part == 0], minlength=ncell)
-    cf_mid = (cf_ptr[:-1].astype(np.int64) + first).astype(np.int32)
+    lf_face = np.ascontiguousarray(geo.lf_face, np.int64)
+    lf_edge = np.ascontiguousarray(geo.lf_edge, np.int64)
+    sgn = np.ascontiguousarray(geo.fe_sign_link, np.float64)
+    F2C = np.ascontiguousarray(geo.F2C, np.int64)
+    nlink = lf_face.shape[0]
+    ncf = nface + int(np.count_nonzero(geo.m1))
+    if max(nlink, ncf) >= 2 ** 31 - 1:
+        raise ValueError("table too large for int32 offsets")
+    curl_ptr, rot_ptr = np.empty(nface + 1, np.int32), np.empty(nedge + 1, np.int32)
+    curl_idx, rot_idx = np.empty(nlink, np.int32), np.empty(nlink, np.int32)
+    cf_ptr, cf_mid = np.empty(ncell + 1, np.int32), np.empty(ncell, np.int32)
+    cf_face, cf_other = np.empty(ncf, np.int32), np.empty(ncf, np.int32)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    N.call("pyib_tables", C.c_int64(nface), C.c_int64(nedge), C.c_int64(ncell), C.c_int64(nlink), p(lf_face),
+           p(lf_edge), p(sgn), p(F2C), p(curl_ptr), p(curl_idx), p(rot_ptr), p(rot_idx), p(cf_ptr), p(cf_mid),
+           p(cf_face), p(cf_other))
+    bf = np.flatnonzero(~np.asarray(geo.m1))
     return Tables(ncell, nface, nedge, int(bf.shape[0]), curl_ptr, curl_idx, rot_ptr, rot_idx, cf_ptr, cf_mid,
-                  face.astype(np.int32), other.astype(np.int32), F2C.astype(np.int32), bf)
+                  cf_face, cf_other, F2C.astype(np.int32), bf)
 
 
 def ghost_kinds(geo, bc_face, bf, cavity, lid_mask=None, zf_mask=None):

@@ -186,6 +186,17 @@ int pyib_geometry(int64_t nnode, const double *nodes, int64_t nedge, const int64
                   double *sgn, double *nf, double *af, double *rf1, double *rf2, double *rf, double *vol_ci,
                   double *aol);
 
+/* Gather tables of the incidence operators (host, setup; the reference's bincount order over
+ * the link list, ib_core_np.py:174-202): curl (face <- its links in link order), rot (edge <-
+ * its links in link order), cell <- faces (first-owner faces, then second-owner faces, each in
+ * face order; cf_mid = start of the second group; boundary face k's opposite is ncell + k).
+ * Signed int32 indices (v >= 0 -> +x[v], v < 0 -> -x[-v-1]).  Returns 1 if a size does not
+ * fit int32. */
+int pyib_tables(int64_t nface, int64_t nedge, int64_t ncell, int64_t nlink, const int64_t *lf_face,
+                const int64_t *lf_edge, const double *sgn, const int64_t *f2c, int32_t *curl_ptr,
+                int32_t *curl_idx, int32_t *rot_ptr, int32_t *rot_idx, int32_t *cf_ptr, int32_t *cf_mid,
+                int32_t *cf_face, int32_t *cf_other);
+
 /* ---- the same CG on the structured layout (ib_driver_np.py:311-410) ----
  * Workspace cell_a/cell_b hold 3*ncell (internal) values; partials >= 4*PYIB_MAX_PARTIALS.
  * flow_mat: w = A p (0 on dir; a caller that drops the Dirichlet rows from the G3 composite
