@@ -16,6 +16,10 @@
 
 #include "../../include/pyib_hip.h"
 
+namespace pyib {
+int fail_msg(const char* msg);   // sets the message pyib_last_error() returns
+}
+
 namespace {
 
 inline double dot3(const double* a, const double* b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
@@ -45,7 +49,8 @@ extern "C" int pyib_geometry(int64_t nnode, const double* nodes, int64_t nedge, 
                              int64_t ncell, int64_t* lf_face, int64_t* lf_edge, double* epos, double* te,
                              double* fpos, double* cpos, double* sgn, double* nf, double* af, double* rf1,
                              double* rf2, double* rf, double* vol_ci, double* aol) {
-  (void)nnode;
+  if (nnode <= 0 || nedge <= 0 || nface <= 0 || ncell <= 0 || nins < 0)
+    return pyib::fail_msg("pyib_geometry: empty mesh");
   const int64_t nlink = 4 * nface + nins;
   for (int64_t f = 0; f < nface; ++f)
     for (int k = 0; k < 4; ++k) {
@@ -175,7 +180,9 @@ extern "C" int pyib_tables(int64_t nface, int64_t nedge, int64_t ncell, int64_t 
                            const int64_t* lf_edge, const double* sgn, const int64_t* f2c, int32_t* curl_ptr,
                            int32_t* curl_idx, int32_t* rot_ptr, int32_t* rot_idx, int32_t* cf_ptr, int32_t* cf_mid,
                            int32_t* cf_face, int32_t* cf_other) {
-  if (nface >= INT32_MAX || nedge >= INT32_MAX || ncell >= INT32_MAX || nlink >= INT32_MAX) return 1;
+  if (nface >= INT32_MAX || nedge >= INT32_MAX || ncell >= INT32_MAX || nlink >= INT32_MAX ||
+      2 * nface >= INT32_MAX)
+    return pyib::fail_msg("pyib_tables: a size does not fit the int32 tables");
   auto enc = [](int64_t v, double s) { return (int32_t)(s > 0.0 ? v : -v - 1); };
   // curl: bucket the links by face
   std::vector<int64_t> pos(nface + 1, 0);

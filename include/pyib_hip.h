@@ -190,8 +190,8 @@ int pyib_geometry(int64_t nnode, const double *nodes, int64_t nedge, const int64
  * the link list, ib_core_np.py:174-202): curl (face <- its links in link order), rot (edge <-
  * its links in link order), cell <- faces (first-owner faces, then second-owner faces, each in
  * face order; cf_mid = start of the second group; boundary face k's opposite is ncell + k).
- * Signed int32 indices (v >= 0 -> +x[v], v < 0 -> -x[-v-1]).  Returns 1 if a size does not
- * fit int32. */
+ * Signed int32 indices (v >= 0 -> +x[v], v < 0 -> -x[-v-1]).  Fails (non-zero, message in
+ * pyib_last_error) if a size does not fit int32. */
 int pyib_tables(int64_t nface, int64_t nedge, int64_t ncell, int64_t nlink, const int64_t *lf_face,
                 const int64_t *lf_edge, const double *sgn, const int64_t *f2c, int32_t *curl_ptr,
                 int32_t *curl_idx, int32_t *rot_ptr, int32_t *rot_idx, int32_t *cf_ptr, int32_t *cf_mid,
