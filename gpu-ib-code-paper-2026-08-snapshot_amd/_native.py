@@ -120,6 +120,9 @@ _SIGS = {
                                 vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "pyib_tables": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_int64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                               vp, vp]),
+    "pyib_mesh_build": (C.c_int, [C.c_int32, vp, vp, vp, vp, C.POINTER(vp), vp]),
+    "pyib_mesh_fetch": (C.c_int, [vp] + [vp] * 15),
+    "pyib_mesh_free": (C.c_int, [vp]),
     "pyib_interior_momentum_adjoint": (C.c_int, [C.c_int64, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                                  C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                                  C.c_double, vp, vp, vp, vp]),

@@ -197,6 +197,25 @@ int pyib_tables(int64_t nface, int64_t nedge, int64_t ncell, int64_t nlink, cons
                 int32_t *curl_idx, int32_t *rot_ptr, int32_t *rot_idx, int32_t *cf_ptr, int32_t *cf_mid,
                 int32_t *cf_face, int32_t *cf_other);
 
+/* ---- nested 2:1 mesh on the host (setup; make_nested_mesh, src/pyib/make_euler_mesh.py:330-463,
+ * with the hanging-node finalisation of make_euler_mesh.py:787-950) ----
+ * Inputs per level k (finest first): shape (nlevels,3) cells, lo (nlevels,3) box corner, dx
+ * (nlevels,3) spacing, hole (nlevels,3) lower corner of level k-1's box in level k's cells (row 0
+ * unused).  pyib_mesh_build keeps the arrays in an opaque handle and returns their counts:
+ * counts[0..6] = nodes, cells, outer boundary quads, faces, edges, interface pairs,
+ * link_fe_insert rows.  pyib_mesh_fetch copies them into host arrays of those sizes (int64
+ * row-major, nodes float64): nodes (n,3), cells (n,8), boundary quads (n,4) and families (n),
+ * faces_to_nodes (n,4), faces_to_cells (n,2), edges_to_nodes (n,2), faces_to_edges (n,4),
+ * bc_node, bc_edge, bc_face, interface_pairs (n,5), link_fe_insert (n,2), cells and new nodes
+ * per level (nlevels).  A NULL output is skipped.  pyib_mesh_free releases the handle. */
+int pyib_mesh_build(int32_t nlevels, const int64_t *shape, const double *lo, const double *dx,
+                    const int64_t *hole, void **handle, int64_t *counts);
+int pyib_mesh_fetch(void *handle, double *nodes, int64_t *cells, int64_t *bfaces, int64_t *bfam, int64_t *f2n,
+                    int64_t *f2c, int64_t *e2n, int64_t *f2e, int64_t *bc_node, int64_t *bc_edge,
+                    int64_t *bc_face, int64_t *pairs, int64_t *insert, int64_t *layer_cells,
+                    int64_t *layer_nodes);
+int pyib_mesh_free(void *handle);
+
 /* ---- the same CG on the structured layout (ib_driver_np.py:311-410) ----
  * Workspace cell_a/cell_b hold 3*ncell (internal) values; partials >= 4*PYIB_MAX_PARTIALS.
  * flow_mat: w = A p (0 on dir; a caller that drops the Dirichlet rows from the G3 composite
