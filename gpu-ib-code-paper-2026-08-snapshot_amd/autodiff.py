@@ -223,22 +223,25 @@ def _b_dinv(K, e):
     return core.interp_f2c(core.curl((e * K.drv.prec).contiguous()))
 
 
-def project_on_range(K, g, tol=1e-13, maxit=20000):
+def project_on_range(K, g, tol=1e-13, maxit=20000, check=25):
     """Component of g in range(A) = range(B^T), in the metric of the Jacobi-PCG:
-    g_r = B^T w with (B D^-1 B^T) w = B D^-1 g (a consistent cell-space system, plain CG)."""
+    g_r = B^T w with (B D^-1 B^T) w = B D^-1 g (a consistent cell-space system, plain CG).
+    The CG scalars stay on the device; the host reads |r|^2 every ``check`` iterations."""
     b = _b_dinv(K, g)
     w = torch.zeros_like(b)
     r = b.clone()
     p = r.clone()
-    rr = float((r * r).sum())
-    stop = (tol ** 2) * rr
+    rr = (r * r).sum()
+    stop = (tol ** 2) * float(rr)
     it = 0
-    while rr > stop and it < maxit:
+    while it < maxit:
+        if it % check == 0 and float(rr) <= stop:
+            break
         q = _b_dinv(K, _bt(K, p))
-        alpha = rr / float((p * q).sum())
-        w.add_(p, alpha=alpha)
-        r.add_(q, alpha=-alpha)
-        rr_new = float((r * r).sum())
+        alpha = rr / (p * q).sum()
+        w.addcmul_(p, alpha)
+        r.addcmul_(q, alpha, value=-1.0)
+        rr_new = (r * r).sum()
         p.mul_(rr_new / rr).add_(r)
         rr = rr_new
         it += 1

@@ -359,8 +359,9 @@ def layout_fingerprint(mesh, markers, widths):
     return d.hexdigest()
 
 
-_SOURCE_FILES = ("case.py", "core.py", "driver.py", "geometry.py", "immersed.py", "interior.py", "kinematics.py",
-                 "mesh.py", "tables.py", "csrc/pyib_hip.hip")
+_SOURCE_FILES = ("case.py", "core.py", "driver.py", "geometry.py", "grid.py", "immersed.py", "interior.py",
+                 "kinematics.py", "layout.py", "mesh.py", "tables.py", "csrc/pyib_hip.hip", "csrc/grid.hip",
+                 "csrc/geometry.cpp", "csrc/mesh.cpp")
 
 
 def source_hashes():
@@ -423,7 +424,7 @@ def build_mesh(config):
         bounds = np.asarray(mc["bounds"], dtype=float)
         _validate_ib_support(config, dx, bounds)
         mesh = M.make_nested_mesh(dx, M.uniform_boxes(bounds), verbose=bool(mc["verbose"]))
-        return mesh, dx, {"kind": kind, "dx": dx, "bounds": bounds.tolist(), "cells": int(mesh.cells.shape[0])}
+        return mesh, dx, mesh_summary(config, mesh)
     ddx = int(mc["cells_per_diameter"])
     dx = 2.0 * radius / ddx
     if kind == "sphere_wake":
@@ -442,10 +443,19 @@ def build_mesh(config):
     mesh = setup_cache.cached("mesh", _mesh_params(config),
                               lambda: M.make_nested_mesh(M.DIAMETER / ddx, boxes, verbose=bool(mc["verbose"])),
                               M.CombinedMesh)
-    summary = {"kind": kind, "diameter": M.DIAMETER, "D_over_dx": ddx, "dx_fine": M.DIAMETER / ddx,
-               "layer_cells": [int(v) for v in mesh.layer_cell_counts], "total_cells": int(mesh.cells.shape[0]),
-               "nodes": int(mesh.nodes.shape[0]), "layer_shape": [list(s) for s in mesh.layer_shape]}
-    return mesh, dx, summary
+    return mesh, dx, mesh_summary(config, mesh)
+
+
+def mesh_summary(config, mesh):
+    """metadata.json's ``mesh`` record (before the face/edge/marker counts are added)."""
+    mc = config["mesh"]
+    if mc["kind"] == "uniform_box":
+        return {"kind": "uniform_box", "dx": float(mc["dx"]), "bounds": np.asarray(mc["bounds"], float).tolist(),
+                "cells": int(mesh.cells.shape[0])}
+    ddx = int(mc["cells_per_diameter"])
+    return {"kind": mc["kind"], "diameter": M.DIAMETER, "D_over_dx": ddx, "dx_fine": M.DIAMETER / ddx,
+            "layer_cells": [int(v) for v in mesh.layer_cell_counts], "total_cells": int(mesh.cells.shape[0]),
+            "nodes": int(mesh.nodes.shape[0]), "layer_shape": [list(s) for s in mesh.layer_shape]}
 
 
 def _mesh_params(config):
@@ -537,7 +547,7 @@ def build_runtime(config, mesh=None, device="cuda", log=None):
         say(f"mesh ready ({summary.get('total_cells', summary.get('cells'))} cells)")
     else:
         dx = float(np.asarray(mesh.dx_uniform)[0])
-        summary = {"kind": config["mesh"]["kind"], "total_cells": int(mesh.cells.shape[0])}
+        summary = mesh_summary(config, mesh)
     precision = config["runtime"]["precision"]
     if precision == "float32":
         raise CaseConfigError("the HIP path runs float64 or mixed precision")

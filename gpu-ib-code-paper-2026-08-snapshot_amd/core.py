@@ -131,28 +131,28 @@ class HipCore:
             raise ValueError(f"expected a contiguous {dtype} device tensor with {n} elements")
 
     def curl(self, edge, out=None):
-        """ib_core_np.py:178-179 (edge -> face)."""
+        """ib_core_np.py:133-134 (edge -> face)."""
         self._check(edge, self.nedge, edge.dtype)
         out = torch.empty(self.nface, dtype=edge.dtype, device=self.device) if out is None else out
         N.call("pyib_curl", self.struct(_prec(edge)), ptr(edge), ptr(out), dev_stream())
         return out
 
     def rot(self, face, out=None):
-        """ib_core_np.py:174-176 (face -> edge)."""
+        """ib_core_np.py:129-131 (face -> edge)."""
         self._check(face, self.nface, face.dtype)
         out = torch.empty(self.nedge, dtype=face.dtype, device=self.device) if out is None else out
         N.call("pyib_rot", self.struct(_prec(face)), ptr(face), ptr(out), dev_stream())
         return out
 
     def cdiv(self, face, out=None):
-        """ib_core_np.py:181-184 (face -> cell)."""
+        """ib_core_np.py:136-139 (face -> cell)."""
         self._check(face, self.nface, face.dtype)
         out = torch.empty(self.ncell, dtype=face.dtype, device=self.device) if out is None else out
         N.call("pyib_cdiv", self.struct(_prec(face)), ptr(face), ptr(out), dev_stream())
         return out
 
     def integr_c2f(self, cellvec, out=None):
-        """ib_core_np.py:189-193 (cell vector -> face); cellvec may carry ghost rows after ncell."""
+        """ib_core_np.py:144-148 (cell vector -> face); cellvec may carry ghost rows after ncell."""
         if not (cellvec.is_cuda and cellvec.is_contiguous() and cellvec.shape[-1] == 3):
             raise ValueError("integr_c2f expects a contiguous (n,3) device tensor")
         out = torch.empty(self.nface, dtype=cellvec.dtype, device=self.device) if out is None else out
@@ -160,14 +160,14 @@ class HipCore:
         return out
 
     def interp_f2c(self, face, out=None):
-        """ib_core_np.py:195-202 (face -> cell vector)."""
+        """ib_core_np.py:150-157 (face -> cell vector)."""
         self._check(face, self.nface, face.dtype)
         out = torch.empty((self.ncell, 3), dtype=face.dtype, device=self.device) if out is None else out
         N.call("pyib_interp_f2c", self.struct(_prec(face)), ptr(face), ptr(out), dev_stream())
         return out
 
     def flow_mat(self, pp, alpd=0.0, visc=0.0, rou=1.0, dt=0.0, out=None, mask_dir=False):
-        """ib_core_np.py:213-220 (matrix-free elliptic operator).  With alpd == 0 the
+        """ib_core_np.py:168-175 (matrix-free elliptic operator).  With alpd == 0 the
         viscous block is the identity (rou=1, visc=0)."""
         self._check(pp, self.nedge, pp.dtype)
         prec = _prec(pp)

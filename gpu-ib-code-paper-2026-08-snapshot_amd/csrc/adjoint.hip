@@ -68,7 +68,7 @@ __global__ void k_zero_idx(int64_t n, const int32_t* __restrict__ idx, double* _
     x[idx[i]] = 0.0;
 }
 
-// integr_c2f^T (ib_core_np.py:189-193 transposed): g_v[c] = sum_{faces with c = c1} g_F rf1
+// integr_c2f^T (ib_core_np.py:144-148 transposed): g_v[c] = sum_{faces with c = c1} g_F rf1
 // - sum_{faces with c = c2} g_F rf2, a cell-centric gather over the cell's face list.
 __global__ void k_integr_adj(int64_t ncell, const int32_t* __restrict__ ptr, const int32_t* __restrict__ mid,
                              const int32_t* __restrict__ face, const double* __restrict__ rf1,

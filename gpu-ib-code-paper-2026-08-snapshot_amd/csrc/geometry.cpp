@@ -1,9 +1,9 @@
 // geometry.cpp -- host-side mesh geometry of the stream-function operators (setup, float64).
 //
-// The quantities of the reference's IBCore.__init__ (ib_core_np.py:68-166): the flat
+// The quantities of the reference's IBCore.__init__ (ib_core_np.py:24-126): the flat
 // face/edge link list (4 links per face + link_fe_insert), edge midpoints and tangents,
 // face and cell centres, the per-link circulation sign, face area vectors, the 2:1
-// "Modify_geom" centre corrections (ib_core_np.py:123-147), rf/rf1/rf2, vol_ci and AoL.
+// "Modify_geom" centre corrections (ib_core_np.py:74-102), rf/rf1/rf2, vol_ci and AoL.
 //
 // Every sum runs in the order of the reference's np.add.at / np.bincount over the link
 // list (and over [c1 owners, then c2 neighbours] for cells), every 3-vector dot product is

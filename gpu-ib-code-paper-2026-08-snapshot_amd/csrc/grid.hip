@@ -2,7 +2,7 @@
 //
 // The CG of Driver.helm_solve (ib_driver_np.py:311-410) runs on the internal
 // layout: per level, dense x/y/z edge blocks and a cell block, +axis oriented.
-// A p = rot(integr(V(interp(curl p)))) (ib_core_np.py:213-220) is applied in
+// A p = rot(integr(V(interp(curl p)))) (ib_core_np.py:168-175) is applied in
 // three passes.  In each pass the constant-coefficient stencil of each level
 // (2.5D tiles staged in LDS, the fastest index k on consecutive lanes) writes
 // the regular rows, and the composite rows (level interfaces, outer boundary,

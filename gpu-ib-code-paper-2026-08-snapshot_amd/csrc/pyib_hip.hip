@@ -49,7 +49,7 @@ __global__ void k_finalize_max(const double* __restrict__ partials, int nb, int 
 }
 
 // ---------------------------------------------------------------- operators
-// curl: U[f] = sum over the face's links of sign * s[e]   (ib_core_np.py:178-179)
+// curl: U[f] = sum over the face's links of sign * s[e]   (ib_core_np.py:133-134)
 template <typename T>
 __global__ void k_curl(int64_t nface, const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
                        const T* __restrict__ s, T* __restrict__ U) {
@@ -60,7 +60,7 @@ __global__ void k_curl(int64_t nface, const int32_t* __restrict__ ptr, const int
   }
 }
 
-// rot: w[e] = sum over the edge's links of sign * F[f]   (ib_core_np.py:174-176)
+// rot: w[e] = sum over the edge's links of sign * F[f]   (ib_core_np.py:129-131)
 // optional: zero on dir_mask, and block partials of p.w
 template <typename T>
 __global__ void k_rot(int64_t nedge, const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
@@ -81,7 +81,7 @@ __global__ void k_rot(int64_t nedge, const int32_t* __restrict__ ptr, const int3
   }
 }
 
-// interp_f2c: vc[c,d] = (sum_{c=c1} U rf1[:,d] - sum_{c=c2} U rf2[:,d]) * vol_ci  (ib_core_np.py:195-202)
+// interp_f2c: vc[c,d] = (sum_{c=c1} U rf1[:,d] - sum_{c=c2} U rf2[:,d]) * vol_ci  (ib_core_np.py:150-157)
 template <typename T>
 __global__ void k_interp(int64_t ncell, const int32_t* __restrict__ ptr, const int32_t* __restrict__ mid,
                          const int32_t* __restrict__ face, const T* __restrict__ rf1, const T* __restrict__ rf2,
@@ -104,7 +104,7 @@ __global__ void k_interp(int64_t ncell, const int32_t* __restrict__ ptr, const i
   }
 }
 
-// viscous block of flow_mat (ib_core_np.py:216-219), three components:
+// viscous block of flow_mat (ib_core_np.py:171-174), three components:
 // y = rou x - (hdt vol_ci) cdiv((AoL visc) cgrad0 x), ghost values zero.
 template <typename T>
 __global__ void k_visc(int64_t ncell, const int32_t* __restrict__ ptr, const int32_t* __restrict__ mid,
@@ -134,7 +134,7 @@ __global__ void k_visc(int64_t ncell, const int32_t* __restrict__ ptr, const int
   }
 }
 
-// integr_c2f: F[f] = vc[c1].rf1 - vc[c2].rf2   (ib_core_np.py:189-193)
+// integr_c2f: F[f] = vc[c1].rf1 - vc[c2].rf2   (ib_core_np.py:144-148)
 template <typename T>
 __global__ void k_integr(int64_t nface, const int32_t* __restrict__ f2c, const T* __restrict__ rf1,
                          const T* __restrict__ rf2, const T* __restrict__ vc, T* __restrict__ F) {
@@ -151,7 +151,7 @@ __global__ void k_integr(int64_t nface, const int32_t* __restrict__ f2c, const T
   }
 }
 
-// cdiv: out[c] = sum_{c=c1} F - sum_{c=c2} F   (ib_core_np.py:181-184)
+// cdiv: out[c] = sum_{c=c1} F - sum_{c=c2} F   (ib_core_np.py:136-139)
 template <typename T>
 __global__ void k_cdiv(int64_t ncell, const int32_t* __restrict__ ptr, const int32_t* __restrict__ mid,
                        const int32_t* __restrict__ face, const T* __restrict__ F, const T* __restrict__ scale,

@@ -30,6 +30,9 @@ from .tables import ghost_kinds
 class HipDriver:
     cg_max_iterations = 1000
     fail_on_cg_nonconvergence = False
+    REFERENCE_TOL1, REFERENCE_TOL2 = 1e-10, 1e-9    # helm_solve's tolerances (ib_driver_np.py:311)
+    max_residual_replacements = 8
+    residual_replacements_last_solve = 0
     cg_timer = None        # optional: object with begin()/end(event, (n, precision)) around PCG blocks
 
     def __init__(self, core, mesh, visc, rou=1.0, ac=0.5, ad=0.5, *, cg_max_iterations=1000,
@@ -394,12 +397,33 @@ class HipDriver:
             self._finish(0, eta0, l2_init, l2_init, err_tot, cg_max)
             return g.to_reference(self._xi), 0
         if self.precision != "mixed":
-            m = self._pcg_loop(m64, w64, self._cgi, self._scal, cg_max, lambda: err_tot, dt, grid=True)
-            N.call("pyib_grid_cg_true_residual", m64, w64, C.byref(self._cgi), ptr(self._resi), ptr(self._si),
-                   ptr(self._tmpi), ptr(self._tri), float(self.rou), float(self.visc), hdt, ptr(self._out2), st())
-            o = self._out2.cpu().numpy()
-            self._finish(m, float(o[0]), float(o[1]), l2_init, err_tot, cg_max)
-            return g.to_reference(self._xi), int(m)
+            # At the reference's tolerances this is its PCG unchanged.  A solve asked for tighter
+            # tolerances (below the recursion residual's drift from the true one) replaces the
+            # residual by the float64 true residual whenever the recursion has passed the stop
+            # test but the true residual has not, keeps the search direction and goes on.
+            # A replacement that does not halve the true residual's eta means the target is below
+            # what float64 can resolve; the solve then ends as the reference's would.
+            replace = tol1 < self.REFERENCE_TOL1 or tol2 < self.REFERENCE_TOL2
+            total, repl, prev = 0, 0, float("inf")
+            while True:
+                total += self._pcg_loop(m64, w64, self._cgi, self._scal, cg_max - total, lambda: err_tot, dt,
+                                        grid=True)
+                N.call("pyib_grid_cg_true_residual", m64, w64, C.byref(self._cgi), ptr(self._resi), ptr(self._si),
+                       ptr(self._tmpi), ptr(self._tri), float(self.rou), float(self.visc), hdt, ptr(self._out2),
+                       st())
+                o = self._out2.cpu().numpy()
+                eta_f, l2_f = float(o[0]), float(o[1])
+                if not (replace and np.isfinite(eta_f) and eta_f > err_tot and total < cg_max
+                        and repl < self.max_residual_replacements and eta_f < 0.5 * prev):
+                    break
+                prev = eta_f
+                repl += 1
+                self._ri.copy_(self._tri)
+                N.call("pyib_cg_replace_residual", ptr(self._scal), ptr(self._out2), st())
+                N.call("pyib_grid_cg_direction", m64, C.byref(self._cgi), st())
+            self.residual_replacements_last_solve = repl
+            self._finish(total, eta_f, l2_f, l2_init, err_tot, cg_max)
+            return g.to_reference(self._xi), int(total)
         # float32 PCG with reliable updates: every time the float32 recurrence residual has
         # dropped by mixed_inner_tol (or reached the float64 target) the float32 correction is
         # added into the float64 solution, the float64 true residual is formed and it replaces

@@ -1,7 +1,7 @@
 """Host-side mesh geometry of the stream-function operators (setup, float64).
 
 Computes the same quantities as the reference ``IBCore.__init__``
-(``src/pyib/solver/ib_core_np.py:68-166``): the flat face/edge link list
+(``src/pyib/solver/ib_core_np.py:24-126``): the flat face/edge link list
 (4 links per face + ``link_fe_insert``), edge midpoints/tangents, face and cell
 centres, the per-link circulation sign, face area vectors, the 2:1
 "Modify_geom" centre corrections, ``rf/rf1/rf2``, ``vol_ci`` and ``AoL``.

@@ -34,6 +34,10 @@ class HipImmersedBoundary:
         self.pos_ref = np.asarray(mesh.pos_ref, dtype=np.float64)
         self.dx = float(np.asarray(mesh.dx_uniform)[0])
         self.hh = float(self.dx if hh is None else hh)
+        # the device stencil has 27 taps (+-1 cell); the delta's support is 1.5 hh, so a wider
+        # kernel (hh > dx) would need the reference's full 5^3 window (ib_immersed.py:35-41)
+        if not 0.0 < self.hh <= self.dx * (1.0 + 1e-12):
+            raise ValueError(f"the HIP stencil needs 0 < hh <= dx (hh={self.hh!r}, dx={self.dx!r})")
         self.ncell = int(mesh.cells.shape[0])
         self.index_cell = idx
         self.device = torch.device(device)

@@ -9,7 +9,7 @@ Slots of the box that are not mesh elements (the hole under a finer level,
 coarse edges collapsed onto fine halves) stay zero.
 
 Away from level boundaries the operator A = rot.integr.V.interp.curl
-(ib_core_np.py:213-220) is a constant-coefficient stencil: the "regular"
+(ib_core_np.py:168-175) is a constant-coefficient stencil: the "regular"
 rows.  They are cells at least ``margin`` cells from the box boundary and
 from the hole, and edges whose four adjacent cells are regular.  A regular
 row is kept only if the mesh geometry around it equals the closed form

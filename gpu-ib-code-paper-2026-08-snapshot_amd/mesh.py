@@ -15,7 +15,7 @@ seconds.  The ordering is bit-identical to the reference, which the sha256
 layout fingerprint of the reference D32/D40 meshes pins.
 
 Only refinement ratio 2 is supported: the geometry correction of the solver
-(``ib_core_np.py:123-147``) is written for ratio 2.
+(``ib_core_np.py:74-102``) is written for ratio 2.
 """
 from __future__ import annotations
 
@@ -156,7 +156,7 @@ def make_nested_mesh(inner_spacing, boxes, refinement_ratio=2, *, verbose=False)
         layer_hole_lo=tuple(tuple(int(v) for v in hh) for hh in hole))
 
 
-# Box lists: sphere_wake_mesh.py:46-59, sphere_quiescent_mesh.py:172-191, plus nested_box.
+# Box lists: sphere_wake_mesh.py:46-59, sphere_quiescent_mesh.py:41-60, plus nested_box.
 WAKE_BOUNDS_D = ((-1.0, 3.0, -1.0, 1.0, -1.0, 1.0), (-2.0, 6.0, -2.0, 2.0, -2.0, 2.0),
                  (-4.0, 12.0, -4.0, 4.0, -4.0, 4.0), (-8.0, 16.0, -8.0, 8.0, -8.0, 8.0))
 CONCENTRIC_HALF_WIDTHS_D = (1.0, 2.0, 4.0, 8.0, 16.0)

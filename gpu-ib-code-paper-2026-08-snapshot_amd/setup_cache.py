@@ -17,7 +17,8 @@ from pathlib import Path
 
 import numpy as np
 
-_SOURCES = ("mesh.py", "geometry.py", "tables.py", "layout.py", "setup_cache.py")
+_SOURCES = ("mesh.py", "geometry.py", "tables.py", "layout.py", "setup_cache.py", "csrc/geometry.cpp",
+            "csrc/mesh.cpp")
 
 
 def cache_dir():
@@ -25,8 +26,10 @@ def cache_dir():
     return Path(d) if d else None
 
 
-def key_for(params: dict) -> str:
-    here = Path(__file__).resolve().parent
+def key_for(params: dict, here: Path = None) -> str:
+    """Key of a cached object: the mesh parameters and the bytes of every source that shapes it
+    (``here`` is the package directory)."""
+    here = Path(__file__).resolve().parent if here is None else Path(here)
     h = hashlib.sha256(json.dumps(params, sort_keys=True).encode())
     for name in _SOURCES:
         h.update((here / name).read_bytes())

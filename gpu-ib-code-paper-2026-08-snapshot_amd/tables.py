@@ -3,7 +3,7 @@
 Each device kernel is a gather over one of these tables, so every output
 element is written by exactly one thread (no atomics) and visits its inputs in
 the same order as the reference's ``bincount`` over the flat link list
-(``ib_core_np.py:174-202``).
+(``ib_core_np.py:129-157``).
 
 * curl  (face <- edge): the face's links in link order (4 per face, then its
   ``link_fe_insert`` links), signed edge index.

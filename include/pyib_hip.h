@@ -31,15 +31,15 @@ extern "C" {
 enum { PYIB_F64 = 0, PYIB_F32 = 1 };
 
 /* Incidence/geometry tables of one mesh, in PyIB's global numbering.
- * Built once on the host from the reference geometry (ib_core_np.py:68-166). */
+ * Built once on the host from the reference geometry (ib_core_np.py:24-126). */
 typedef struct pyib_mesh {
   int32_t dtype;            /* PYIB_F64 or PYIB_F32: type of every float array below */
   int32_t reserved;
   int64_t ncell, nface, nedge, nghost;
-  /* curl: face <- edges over the flat face/edge links (ib_core_np.py:178-179) */
+  /* curl: face <- edges over the flat face/edge links (ib_core_np.py:133-134) */
   const int32_t *curl_ptr;  /* nface+1 */
   const int32_t *curl_idx;  /* signed edge index per link, link order */
-  /* rot: edge <- faces, the transpose of curl (ib_core_np.py:174-176) */
+  /* rot: edge <- faces, the transpose of curl (ib_core_np.py:129-131) */
   const int32_t *rot_ptr;   /* nedge+1 */
   const int32_t *rot_idx;   /* signed face index */
   /* cell <- face incidence: for cell c, entries [cf_ptr[c], cf_mid[c]) are the
@@ -127,13 +127,13 @@ typedef struct pyib_grid {
 const char *pyib_last_error(void);
 int pyib_device_count(int *count);
 
-/* ---- incidence operators (ib_core_np.py:174-228) ---- */
+/* ---- incidence operators (ib_core_np.py:129-183) ---- */
 int pyib_curl(const pyib_mesh *m, const void *edge, void *face, void *stream);          /* IBCore.curl */
 int pyib_rot(const pyib_mesh *m, const void *face, void *edge, void *stream);           /* IBCore.rot */
 int pyib_interp_f2c(const pyib_mesh *m, const void *face, void *cellvec, void *stream); /* IBCore.interp_f2c */
 int pyib_integr_c2f(const pyib_mesh *m, const void *cellvec, void *face, void *stream); /* IBCore.integr_c2f */
 int pyib_cdiv(const pyib_mesh *m, const void *face, void *cell, void *stream);          /* IBCore.cdiv */
-/* IBCore.flow_mat + DirBC (ib_core_np.py:213-220, ib_driver_np.py:387-388):
+/* IBCore.flow_mat + DirBC (ib_core_np.py:168-175, ib_driver_np.py:387-388):
  * w = rot(integr(V(interp(curl p)))), V x = rou x - hdt*vol_ci*cdiv(AoL*visc*cgrad0 x),
  * hdt = 0.5*dt; w is zeroed on dir_mask when mask_dir != 0.  If dot_out is
  * non-NULL, *dot_out (device) receives p.w in double (deterministic). */
@@ -173,7 +173,7 @@ int pyib_cg_true_residual(const pyib_mesh *m, const pyib_work *ws, const pyib_cg
 int pyib_extrapolate(int64_t n, const double *s, const double *s_old, double dt, double dt_old,
                      double *out, void *stream);
 
-/* ---- mesh geometry on the host (setup; IBCore.__init__, ib_core_np.py:68-166) ----
+/* ---- mesh geometry on the host (setup; IBCore.__init__, ib_core_np.py:24-126) ----
  * Host pointers, float64 / int64, row-major.  Inputs: nodes (nnode,3), e2n (nedge,2),
  * f2c (nface,2; -1 = no neighbour), f2e (nface,4), ins (nins,2) = link_fe_insert (face, edge).
  * Outputs (preallocated): lf_face, lf_edge (nlink = 4 nface + nins), epos, te (nedge,3),
@@ -187,7 +187,7 @@ int pyib_geometry(int64_t nnode, const double *nodes, int64_t nedge, const int64
                   double *aol);
 
 /* Gather tables of the incidence operators (host, setup; the reference's bincount order over
- * the link list, ib_core_np.py:174-202): curl (face <- its links in link order), rot (edge <-
+ * the link list, ib_core_np.py:129-157): curl (face <- its links in link order), rot (edge <-
  * its links in link order), cell <- faces (first-owner faces, then second-owner faces, each in
  * face order; cf_mid = start of the second group; boundary face k's opposite is ncell + k).
  * Signed int32 indices (v >= 0 -> +x[v], v < 0 -> -x[-v-1]).  Fails (non-zero, message in

@@ -10,7 +10,7 @@ import numpy as np
 
 
 def _avg_to_cells(F2C, m1, ncell, fvals):
-    """Cell average of face quantities over both owners (ib_core_np.py:95-98, 123-127)."""
+    """Cell average of face quantities over both owners (ib_core_np.py:50-53, 78-82)."""
     s = np.zeros((ncell, 3))
     c = np.zeros(ncell)
     np.add.at(s, F2C[:, 0], fvals)
@@ -21,7 +21,7 @@ def _avg_to_cells(F2C, m1, ncell, fvals):
 
 
 class Core:
-    """ib_core_np.py:68-171 (IBCore) in float64."""
+    """ib_core_np.py:24-126 (IBCore) in float64."""
 
     def __init__(self, mesh, ND=3, geometry=None):
         self.ND = ND
@@ -41,11 +41,11 @@ class Core:
         self.nface = nface = F2C.shape[0]
         self.nedge = E2N.shape[0]
         self.m1 = m1 = F2C[:, 1] >= 0
-        # ib_core_np.py:84-86 -- flat face/edge links: 4 per face + link_fe_insert
+        # ib_core_np.py:39-41 -- flat face/edge links: 4 per face + link_fe_insert
         ins = np.asarray(getattr(mesh, "link_fe_insert", np.zeros((0, 2), np.int64)))
         self.lf_face = lf_face = np.concatenate([np.repeat(np.arange(nface), 4), ins[:, 0]])
         self.lf_edge = lf_edge = np.concatenate([F2E.ravel(), ins[:, 1]])
-        # ib_core_np.py:89-98 -- edge midpoint/tangent, face and cell centres
+        # ib_core_np.py:44-53 -- edge midpoint/tangent, face and cell centres
         self.epos = epos = nodes[E2N].mean(axis=1)
         self.te = te = nodes[E2N[:, 1]] - nodes[E2N[:, 0]]
         fs = np.zeros((nface, 3))
@@ -54,11 +54,11 @@ class Core:
         np.add.at(fc, lf_face, 1.0)
         fpos = fs / fc[:, None]
         cpos = _avg_to_cells(F2C, m1, ncell, fpos)
-        # ib_core_np.py:101-107 -- pre-correction rf
+        # ib_core_np.py:55-62 -- pre-correction rf
         rf = np.zeros((nface, 3))
         rf[m1] = cpos[F2C[m1, 1]] - cpos[F2C[m1, 0]]
         rf[~m1] = fpos[~m1] - cpos[F2C[~m1, 0]]
-        # ib_core_np.py:110-117 -- per-link circulation sign and face area vector
+        # ib_core_np.py:64-72 -- per-link circulation sign and face area vector
         avec = np.cross(epos[lf_edge] - fpos[lf_face], te[lf_edge])
         sgn = np.sign((avec * rf[lf_face]).sum(axis=1))
         sgn[sgn == 0] = 1.0
@@ -68,7 +68,7 @@ class Core:
         nf *= 1.0 / (ND - 1)
         self.nf = nf
         self.Af = Af = np.linalg.norm(nf, axis=1)
-        # ib_core_np.py:123-147 -- Modify_geom (2:1 hanging-node centre corrections)
+        # ib_core_np.py:74-102 -- Modify_geom (2:1 hanging-node centre corrections)
         tol = 1e-6 * Af
         rfe = epos[lf_edge] - fpos[lf_face]
         hitf = np.abs((rfe * rfe).sum(1) - 0.36 * Af[lf_face]) < tol[lf_face]
@@ -87,13 +87,13 @@ class Core:
         self.rf2 = rf2 = np.zeros((nface, 3))
         rf2[m1] = fpos[m1] - cpos[F2C[m1, 1]]
         self.rf = rf = rf1 - rf2
-        # ib_core_np.py:150-156 -- inverse cell volume and diffusion face factor
+        # ib_core_np.py:104-111 -- inverse cell volume and diffusion face factor
         vraw = np.zeros(ncell)
         np.add.at(vraw, F2C[:, 0], (nf * rf1).sum(axis=1))
         np.add.at(vraw, F2C[m1, 1], -(nf[m1] * rf2[m1]).sum(axis=1))
         self.vol_ci = np.where(np.abs(vraw) > 1e-14, ND / vraw, 0.0)
         self.AoL = (rf * nf).sum(axis=1) / (rf * rf).sum(axis=1)
-        # ib_core_np.py:160-166 -- gradient-reconstruction matrix (self-test only)
+        # ib_core_np.py:113-121 -- gradient-reconstruction matrix (self-test only)
         contrib = 0.5 * nf[:, :, None] * rf[:, None, :]
         matr = np.zeros((ncell, 3, 3))
         np.add.at(matr, F2C[:, 0], contrib)
@@ -101,7 +101,7 @@ class Core:
         matr[self.vol_ci <= 0] = np.eye(3)
         self.matr = matr
 
-    # ib_core_np.py:174-228 -------------------------------------------------
+    # ib_core_np.py:129-183 -------------------------------------------------
     def rot(self, face):
         return np.bincount(self.lf_edge, self.fe_sign_link * face[self.lf_face], minlength=self.nedge)
 
@@ -145,7 +145,7 @@ class Core:
         return out
 
     def flow_mat(self, pp, alpd=0.0, visc=0.0, rou=1.0, dt=0.0):
-        """ib_core_np.py:213-220; the viscous factor is the literal 0.5."""
+        """ib_core_np.py:168-175; the viscous factor is the literal 0.5."""
         vcq = self.interp_f2c(self.curl(pp))
         if alpd != 0.0:
             for k in range(self.ND):

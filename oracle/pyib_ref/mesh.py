@@ -393,7 +393,7 @@ def make_nested_mesh(h, boxes, refinement_ratio=2):
                 interface_pairs=pairs, link_fe_insert=insert)
 
 
-# sphere_wake_mesh.py:42-51 and sphere_quiescent_mesh.py:170-172
+# sphere_wake_mesh.py:42-51 and sphere_quiescent_mesh.py:39-41
 DIAMETER = 0.5
 WAKE_BOUNDS_D = ((-1.0, 3.0, -1.0, 1.0, -1.0, 1.0), (-2.0, 6.0, -2.0, 2.0, -2.0, 2.0),
                  (-4.0, 12.0, -4.0, 4.0, -4.0, 4.0), (-8.0, 16.0, -8.0, 8.0, -8.0, 8.0))
@@ -406,7 +406,7 @@ def wake_boxes():
 
 
 def concentric_boxes():
-    """sphere_quiescent_mesh.py:175-191."""
+    """sphere_quiescent_mesh.py:44-60."""
     out = []
     for hw_d in CONCENTRIC_HALF_WIDTHS_D:
         hw = DIAMETER * hw_d

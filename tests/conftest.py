@@ -10,6 +10,8 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
 def pytest_configure(config):
+    import tests.helpers as H
+    H.CAPTURE_MANAGER = config.pluginmanager.getplugin("capturemanager")
     config.addinivalue_line("markers", "gpu: needs a ROCm GPU (MI355X) and the built HIP library")
     config.addinivalue_line("markers", "slow: takes more than ~20 s on the CPU")
 

@@ -8,8 +8,8 @@ modules are absent from this image, so empty placeholder modules are put in
 Fixtures written to tests/golden/ (NumPy .npz, loaded with allow_pickle=False):
 
 * ``geometry_2level.npz`` -- IBCore geometry of a 2-level 2:1 nested mesh
-  (Modify_geom interface corrections), ib_core_np.py:68-166.
-* ``flow_mat_2level.npz`` -- flow_mat and CSR A@p of a seeded p (ib_core_np.py:213-220,
+  (Modify_geom interface corrections), ib_core_np.py:24-126.
+* ``flow_mat_2level.npz`` -- flow_mat and CSR A@p of a seeded p (ib_core_np.py:168-175,
   ib_sparse.py:69-96).
 * ``traj_<case>.npz`` -- short trajectories of the bundled example cases
   (ib_case.run_case): s, s_old, vc, forces after the run, plus every history row.

@@ -1,5 +1,5 @@
 """The product's mesh geometry (C++ ``pyib_geometry``, host only, no GPU) is bitwise the
-oracle's NumPy restatement of ``IBCore.__init__`` (ib_core_np.py:68-166) on 2:1 nested
+oracle's NumPy restatement of ``IBCore.__init__`` (ib_core_np.py:24-126) on 2:1 nested
 meshes, including the interface centre corrections."""
 import numpy as np
 import pytest

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 import pyib_hip  # noqa: E402,F401
 import pyib_hip.autodiff as AD  # noqa: E402
 import pyib_hip.case as HC  # noqa: E402
-from tests.helpers import small_case  # noqa: E402
+from tests.helpers import progress, small_case  # noqa: E402
 
 
 def _dot(a, b):
@@ -173,7 +173,8 @@ def _objective(steps, every, tol1=1e-14, tol2=1e-13):
     cfg = small_case("oscillating", steps=steps)
     rt = HC.build_runtime(cfg)
     HC.initial_state(rt)
-    return AD.PeriodForceObjective(rt, steps, checkpoint_every=every, tol1=tol1, tol2=tol2, cg_max=20000), rt
+    return AD.PeriodForceObjective(rt, steps, checkpoint_every=every, tol1=tol1, tol2=tol2, cg_max=20000,
+                                   log=lambda m: progress(f"[gpu] adjoint: {m}")), rt
 
 
 def test_forward_objective_matches_runner():

@@ -203,3 +203,63 @@ def test_mixed_precision_force_harmonics_over_a_period():
     err = float(np.linalg.norm(hmx - h64) / np.linalg.norm(h64))
     print(f"[gpu] one period, mixed vs float64: harmonic (a, b, c) = {h64} vs {hmx}, rel err {err:.2e}")
     assert err <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["cpu_fixed_sphere_smoke", "cpu_oscillating_sphere_smoke"])
+def test_run_case_history_matches_reference_and_validates(tmp_path, name):
+    """run_case on the reference's bundled smoke case (its own case file, device hip, final field
+    saved): the history.csv it writes matches the reference's own run column by column
+    (tests.helpers.compare_history), and the artifact validator accepts the directory;
+    tampered copies are refused with the reason."""
+    import shutil
+    import pyib_hip.validate as V
+    from tests.helpers import compare_history, golden, golden_config
+    cfg = golden_config(name)
+    cfg["runtime"]["device"] = "hip"
+    cfg["solver"]["sparse"] = False
+    cfg["output"]["save_field"] = True
+    cfg = HC.validate_config(cfg)
+    out = tmp_path / "run"
+    HC.run_case(cfg, out, quiet=True)
+    rows = HC.read_history(out / "history.csv")
+    with golden(f"traj_{name}.npz") as g:
+        report = compare_history(rows, g["history"], [str(c) for c in g["columns"]])
+    print(f"[gpu] {name} run_case history.csv vs the reference's, worst / allowance:",
+          {k: round(v, 3) for k, v in sorted(report.items(), key=lambda kv: -kv[1])[:5]})
+    summary = V.validate_result(out)
+    assert summary["coverage"] == "full_field" and summary["completed_step"] == cfg["time"]["steps"]
+    assert V.main([str(out)]) == 0
+
+    def tampered(tag, edit):
+        d = tmp_path / tag
+        shutil.copytree(out, d)
+        edit(d)
+        with pytest.raises(V.ValidationError) as e:
+            V.validate_result(d)
+        return str(e.value)
+
+    def device(d):
+        with np.load(d / "checkpoint.npz", allow_pickle=False) as a:
+            arr = {k: a[k] for k in a.files}
+        arr["device"] = np.str_("cuda")
+        np.savez(d / "checkpoint.npz", **arr)
+
+    def sources(d):
+        md = json.loads((d / "metadata.json").read_text())
+        md["source_sha256"]["driver.py"] = "0" * 64
+        (d / "metadata.json").write_text(json.dumps(md))
+
+    def history(d):
+        lines = (d / "history.csv").read_text().splitlines()
+        (d / "history.csv").write_text("\n".join(lines[:-2] + lines[-1:]) + "\n")
+
+    def field(d):
+        with np.load(d / "final_field.npz", allow_pickle=False) as a:
+            arr = {k: a[k] for k in a.files}
+        arr["ib_force_E"] = arr["ib_force_E"] * (1.0 + 1e-6)
+        np.savez(d / "final_field.npz", **arr)
+
+    assert "device" in tampered("device", device)
+    assert "source hash" in tampered("sources", sources)
+    assert "omits required sampled steps" in tampered("history", history)
+    assert "differs between checkpoint and final field" in tampered("field", field)

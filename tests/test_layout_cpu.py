@@ -2,7 +2,7 @@
 
 A NumPy emulation of the structured stencils on the regular rows plus the
 composite rows of the irregular ones must reproduce the oracle's flow_mat
-(ib_core_np.py:213-220) through the signed renumbering, on 2:1 nested meshes
+(ib_core_np.py:168-175) through the signed renumbering, on 2:1 nested meshes
 with dyadic and non-dyadic spacing.
 """
 import numpy as np

@@ -141,7 +141,7 @@ def test_cg_reaches_dense_solution():
 
 
 def test_grad_recon_linear_exact(core2):
-    """ib_core_np.py:240-252 self-test: the gradient of a linear field is e_k."""
+    """ib_core_np.py:186-209 self-test: the gradient of a linear field is e_k."""
     c = core2
     for k in range(3):
         phi = c.cpos[:, k]

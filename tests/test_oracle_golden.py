@@ -33,3 +33,24 @@ def test_oracle_reproduces_reference_trajectory(name, steps):
         if "vc" in g.files:
             assert rel(vc[:rt.core.ncell], g["vc"]) < 1e-12
             assert rel(rt.driver.ib_force_E, g["ib_force_E"]) < 1e-10
+
+
+@pytest.mark.parametrize("name,steps", [("cpu_fixed_sphere_smoke", 5), ("cpu_oscillating_sphere_smoke", 4),
+                                        ("cpu_fixed_sphere_re100_d8_gate", 3)])
+def test_matrix_free_oracle_meets_the_history_rules(name, steps):
+    """The oracle's matrix-free path (the one the HIP path is compared with; the reference ran
+    its CSR matrix) meets every per-column rule of tests.helpers.compare_history against the
+    reference's history, and a planted error in one column is caught."""
+    from tests.helpers import compare_history
+    cfg = golden_config(name)
+    rt = OC.build_runtime(cfg, sparse=False)
+    rows = OC.run(rt, steps)[3]
+    with golden(f"traj_{name}.npz") as g:
+        cols = [str(c) for c in g["columns"]]
+        hist = g["history"]
+    report = compare_history(rows, hist, cols)
+    print(name, {k: f"{v:.2g}" for k, v in report.items() if v > 1e-3})
+    bad = [dict(r) for r in rows]
+    bad[-1]["slip_max"] *= 1.0 + 1e-8
+    with pytest.raises(AssertionError, match="slip_max"):
+        compare_history(bad, hist, cols)
