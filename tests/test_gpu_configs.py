@@ -64,6 +64,8 @@ def test_cfg1_against_the_oracle_run():
     and the edge vorticity (on every 32nd edge) within 3x the oracle's own CSR-vs-matrix-free
     difference (or 1e-12)."""
     path = os.path.join(GOLDEN, "cfg1_oracle.npz")
+    if not os.path.exists(path):
+        pytest.skip("tests/golden/cfg1_oracle.npz has not been written (make_cfg1_fixture.py)")
     with np.load(path, allow_pickle=False) as g:
         fx = {k: g[k] for k in g.files}
     cfg = json.loads(str(fx["config"]))
