@@ -32,6 +32,7 @@ class Mesh(C.Structure):
 
 
 MAX_LEVELS = 8
+IB_NOALLOC_MAX_MARKERS = 16384     # PYIB_IB_NOALLOC_MAX_MARKERS of include/pyib_hip.h
 
 
 class Level(C.Structure):
@@ -101,6 +102,7 @@ _SIGS = {
     "pyib_ib_interp": (C.c_int, [C.c_int, C.c_int64, vp, vp, C.c_double, vp, vp, vp]),
     "pyib_ib_sort_temp_bytes": (C.c_int, [C.c_int64, C.POINTER(C.c_int64)]),
     "pyib_ib_sort": (C.c_int, [C.c_int64, vp, vp, vp, vp, vp, vp, C.c_int64, vp]),
+    "pyib_ib_sort_noalloc": (C.c_int, [C.c_int64, vp, vp, vp, vp]),
     "pyib_ib_spread": (C.c_int, [C.c_int, C.c_int64, vp, vp, vp, vp, vp, C.c_double, C.c_int, vp, vp]),
     "pyib_ib_force": (C.c_int, [C.c_int64, vp, vp, C.c_double, vp, vp]),
     "pyib_interior_momenta": (C.c_int, [C.c_int, C.c_int64, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,

@@ -12,6 +12,7 @@
 #include <hipcub/hipcub.hpp>
 #include <cstring>
 #include <limits>
+#include <mutex>
 
 #include "common.h"
 
@@ -620,6 +621,41 @@ int cg_iterate_t(const pyib_mesh* m, const pyib_work* ws, const pyib_cg* cg, dou
 
 }  // namespace
 
+// ------------------------------------------- allocation-free marker sort: device scratch
+// One file-scope array per GPU the library runs on, held until the process ends.  Layout:
+// key and value staging of up to kNoAllocMaxTaps entries each, then the radix sort's temporary
+// storage.  The host never uses the array's name as an address: the address is fetched once per
+// device with hipGetSymbolAddress (the name is only a host-side stand-in in host code).
+constexpr int64_t kNoAllocMaxMarkers = PYIB_IB_NOALLOC_MAX_MARKERS;
+constexpr int64_t kNoAllocMaxTaps = kNoAllocMaxMarkers * 27;
+constexpr int64_t kNoAllocStage = ((kNoAllocMaxTaps * 4 + 255) / 256) * 256;   // bytes per staging array
+constexpr int64_t kNoAllocBytes = PYIB_IB_NOALLOC_SCRATCH_BYTES;
+static_assert(kNoAllocBytes > 2 * kNoAllocStage, "scratch too small for its staging arrays");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// the current device's copy of `scratch`, looked up on the first call made on that device
+int scratch_for_current_device(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return pyib::fail("pyib_ib_sort_noalloc: hipGetDevice", e);
+  if (dev < 0 || dev >= kMaxDevices) return pyib::fail_msg("pyib_ib_sort_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return pyib::fail("pyib_ib_sort_noalloc: hipGetSymbolAddress(scratch)", e);
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
 // ====================================================================== C ABI
 extern "C" {
 using namespace pyib;
@@ -880,6 +916,34 @@ int pyib_ib_sort(int64_t nmark, const int32_t* cell, int32_t* keys_tmp, int32_t*
                                                     (int)(sizeof(int32_t) * 8), st);
   if (e != hipSuccess) return fail("pyib_ib_sort", e);
   return check_launch("pyib_ib_sort");
+}
+
+int pyib_ib_sort_noalloc(int64_t nmark, const int32_t* cell, int32_t* keys_sorted, int32_t* vals_sorted,
+                         void* stream) {
+  if (nmark < 0 || nmark > kNoAllocMaxMarkers) {
+    std::snprintf(g_err, sizeof(g_err), "pyib_ib_sort_noalloc: %lld markers outside [0, %lld] (the static scratch's "
+                  "limit; use pyib_ib_sort with caller workspaces)", (long long)nmark, (long long)kNoAllocMaxMarkers);
+    return 2;
+  }
+  if (nmark == 0) return 0;
+  void* base = nullptr;
+  if (int rc = scratch_for_current_device(&base)) return rc;
+  const int64_t n = nmark * 27;
+  size_t need = 0;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, need, (const int32_t*)nullptr, (int32_t*)nullptr,
+                                                    (const int32_t*)nullptr, (int32_t*)nullptr, (int)n);
+  if (e != hipSuccess) return fail("pyib_ib_sort_noalloc", e);
+  const size_t avail = (size_t)(kNoAllocBytes - 2 * kNoAllocStage);
+  if (need > avail) {
+    std::snprintf(g_err, sizeof(g_err), "pyib_ib_sort_noalloc: radix-sort storage %zu bytes exceeds the %zu in the "
+                  "static scratch", need, avail);
+    return 2;
+  }
+  unsigned char* b = static_cast<unsigned char*>(base);
+  int32_t* keys_tmp = reinterpret_cast<int32_t*>(b);
+  int32_t* vals_tmp = reinterpret_cast<int32_t*>(b + kNoAllocStage);
+  void* temp = b + 2 * kNoAllocStage;
+  return pyib_ib_sort(nmark, cell, keys_tmp, vals_tmp, keys_sorted, vals_sorted, temp, (int64_t)avail, stream);
 }
 
 int pyib_ib_spread(int dtype, int64_t nmark, const int32_t* keys_sorted, const int32_t* vals_sorted, const double* w,

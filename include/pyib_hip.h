@@ -314,6 +314,18 @@ int pyib_ib_sort_temp_bytes(int64_t n, int64_t *bytes);
 int pyib_ib_sort(int64_t nmark, const int32_t *cell, int32_t *keys_tmp, int32_t *vals_tmp,
                  int32_t *keys_sorted, int32_t *vals_sorted, void *temp, int64_t temp_bytes,
                  void *stream);
+/* Allocation-free variant of pyib_ib_sort.  It has no counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops).  The key/value staging and the radix-sort storage live in a static device array of
+ * PYIB_IB_NOALLOC_SCRATCH_BYTES bytes per GPU (held until the process ends); nmark may be at
+ * most PYIB_IB_NOALLOC_MAX_MARKERS (larger counts fail with a message and launch nothing).
+ * The caller passes only the inputs and the sorted outputs, which equal pyib_ib_sort's.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (two streams,
+ * two threads) race and must be serialised by the caller. */
+#define PYIB_IB_NOALLOC_MAX_MARKERS 16384
+#define PYIB_IB_NOALLOC_SCRATCH_BYTES (16 * 1024 * 1024)
+int pyib_ib_sort_noalloc(int64_t nmark, const int32_t *cell, int32_t *keys_sorted, int32_t *vals_sorted,
+                         void *stream);
 int pyib_ib_spread(int dtype, int64_t nmark, const int32_t *keys_sorted, const int32_t *vals_sorted,
                    const double *w, const double *flag, const double *ds3, double scale,
                    int accumulate, void *out, void *stream);
