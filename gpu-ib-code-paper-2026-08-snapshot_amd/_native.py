@@ -56,6 +56,12 @@ class Work(C.Structure):
     _fields_ = [("face", vp), ("cell_a", vp), ("cell_b", vp), ("partials", vp)]
 
 
+class PGraph(C.Structure):
+    _fields_ = [("ncell", C.c_int64), ("nface", C.c_int64), ("fc", vp), ("pos", vp), ("grad", vp), ("area", vp),
+                ("ptr", vp), ("nbr", vp), ("sface", vp), ("gauge", vp), ("w", vp), ("jmp", vp), ("dinv", vp),
+                ("x", vp), ("r", vp), ("z", vp), ("p", vp), ("q", vp), ("scal", vp), ("partials", vp)]
+
+
 class CG(C.Structure):
     _fields_ = [("x", vp), ("r", vp), ("p", vp), ("w", vp), ("prec", vp), ("scal", vp)]
 
@@ -122,6 +128,10 @@ _SIGS = {
                                 vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "pyib_tables": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_int64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                               vp, vp]),
+    "pyib_pressure_graph": (C.c_int, [C.c_int64, C.c_int64, vp, vp, vp, vp]),
+    "pyib_pressure_begin": (C.c_int, [C.POINTER(PGraph), vp]),
+    "pyib_pressure_iterate": (C.c_int, [C.POINTER(PGraph), C.c_int32, vp]),
+    "pyib_pressure_finish": (C.c_int, [C.POINTER(PGraph), vp, vp]),
     "pyib_mesh_build": (C.c_int, [C.c_int32, vp, vp, vp, vp, C.POINTER(vp), vp]),
     "pyib_mesh_fetch": (C.c_int, [vp] + [vp] * 15),
     "pyib_mesh_free": (C.c_int, [vp]),

@@ -197,6 +197,34 @@ int pyib_tables(int64_t nface, int64_t nedge, int64_t ncell, int64_t nlink, cons
                 int32_t *curl_idx, int32_t *rot_ptr, int32_t *rot_idx, int32_t *cf_ptr, int32_t *cf_mid,
                 int32_t *cf_face, int32_t *cf_other);
 
+/* ---- offline pressure recovery of a saved field (recover_pressure, ib_postprocess.py:49-105) ----
+ * The cell pressure whose face jumps best fit the saved gradient: L p = I^T W j on the cell graph
+ * (w_f = area / |x2 - x1|, j_f = (g1 + g2)/2 . (x2 - x1) on internal faces), cell 0 pinned,
+ * Jacobi-preconditioned CG, then the mean over the gauge cells removed.
+ * pyib_pressure_graph (host) builds the cell-major adjacency: for cell c, entries [ptr[c],
+ * ptr[c+1]) are its internal faces in face order, nbr the other cell, sface the face index
+ * (f if c is the face's second cell, -f-1 if its first); arrays of 2 x (internal faces).
+ * pyib_pressure_begin forms w, j, the right-hand side, D^-1 and the CG start; iterate runs
+ * n_iter CG iterations with the scalars on the device (scal[4] = r.r, scal[5] = b.b, the host
+ * stops when r.r < tol^2 b.b); finish removes the gauge mean from x and writes out4 = [sum of x
+ * over the gauge cells, their count, sum w (p2 - p1 - j)^2, sum w]. */
+typedef struct pyib_pgraph {
+  int64_t ncell, nface;
+  const int32_t *fc;                       /* nface x 2, second cell -1 on boundary faces */
+  const double *pos, *grad, *area;         /* ncell x 3, ncell x 3, nface */
+  const int32_t *ptr, *nbr, *sface;        /* adjacency of pyib_pressure_graph */
+  const uint8_t *gauge;                    /* ncell: 1 on the gauge cells */
+  double *w, *jmp;                         /* nface */
+  double *dinv, *x, *r, *z, *p, *q;        /* ncell; x is the pressure */
+  double *scal;                            /* 8 */
+  double *partials;                        /* >= 2 * PYIB_MAX_PARTIALS */
+} pyib_pgraph;
+int pyib_pressure_graph(int64_t ncell, int64_t nface, const int64_t *face_cells, int32_t *ptr, int32_t *nbr,
+                        int32_t *sface);
+int pyib_pressure_begin(const pyib_pgraph *g, void *stream);
+int pyib_pressure_iterate(const pyib_pgraph *g, int32_t n_iter, void *stream);
+int pyib_pressure_finish(const pyib_pgraph *g, double *out4, void *stream);
+
 /* ---- nested 2:1 mesh on the host (setup; make_nested_mesh, src/pyib/make_euler_mesh.py:330-463,
  * with the hanging-node finalisation of make_euler_mesh.py:787-950) ----
  * Inputs per level k (finest first): shape (nlevels,3) cells, lo (nlevels,3) box corner, dx

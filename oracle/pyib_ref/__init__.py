@@ -21,5 +21,9 @@ Pinning (what makes this oracle trustworthy), by module:
                    spread/interp duality.
 * ``kinematics``-- analytic velocity == central finite difference.
 * ``interior``  -- closed-form sphere indicator volume; golden trajectories.
-* ``case``      -- golden history rows; checkpoint/config fingerprints.
+* ``case``      -- golden history rows; checkpoint/config fingerprints; the harmonic fit
+                   against the reference's published D32/D40 fits (expected.json).
+* ``postprocess``-- the reference's own manufactured-pressure and disconnected-graph tests
+                   (tests/test_ib_postprocess.py:76-85, 113-116), restated; dense solve of
+                   the pinned graph system on small graphs.
 """

@@ -54,7 +54,7 @@ def test_load_fails_loudly_when_missing(tmp_path):
 def test_struct_offsets_match_the_c_compiler(tmp_path):
     """Every ctypes mirror of a C-ABI struct has the C compiler's size and field offsets."""
     structs = {"pyib_mesh": N.Mesh, "pyib_work": N.Work, "pyib_cg": N.CG, "pyib_level": N.Level,
-               "pyib_grid": N.Grid}
+               "pyib_grid": N.Grid, "pyib_pgraph": N.PGraph}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pyib_hip.h"', "int main(void) {"]
     for cname, cls in structs.items():
         lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
