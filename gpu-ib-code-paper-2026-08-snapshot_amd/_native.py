@@ -110,6 +110,7 @@ _SIGS = {
     "pyib_ib_sort": (C.c_int, [C.c_int64, vp, vp, vp, vp, vp, vp, C.c_int64, vp]),
     "pyib_ib_sort_noalloc": (C.c_int, [C.c_int64, vp, vp, vp, vp]),
     "pyib_ib_spread": (C.c_int, [C.c_int, C.c_int64, vp, vp, vp, vp, vp, C.c_double, C.c_int, vp, vp]),
+    "pyib_ib_coupling": (C.c_int, [C.c_int64, vp, vp, vp, vp, vp, C.c_double, vp, vp]),
     "pyib_ib_force": (C.c_int, [C.c_int64, vp, vp, C.c_double, vp, vp]),
     "pyib_interior_momenta": (C.c_int, [C.c_int, C.c_int64, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                         C.c_double, C.c_double, C.c_double, C.c_int, vp, vp, vp, vp]),

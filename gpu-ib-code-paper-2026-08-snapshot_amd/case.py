@@ -592,8 +592,7 @@ def build_runtime(config, mesh=None, device="cuda", log=None):
     else:
         drv.set_ib(ib, markers, widths, np.zeros((count, 3)), force_mode=mode)
     drv.ib_alpha = float(ibc["alpha"])
-    if bool(ibc["implicit"]):
-        raise CaseConfigError("implicit IB forcing is not part of the HIP path")
+    drv.ib_implicit = bool(ibc["implicit"])      # ib_case.py:766
     mt = float(mo["amplitude"]) * np.abs(np.asarray(mo["axis"], dtype=float)) if mo["kind"] == "oscillation" \
         else np.zeros(3)
     load = HipLoadCorrection(core.cpos[:core.ncell], 1.0 / core.vol_ci, reference_center=centre,

@@ -464,13 +464,21 @@ class HipDriver:
         self.ib_force_E.zero_()
         ib.spread_into(self.ib_force_E, self.ib_force_L, self._ds3, 1.0, accumulate=False)
 
+    def _lagrangian_force(self, vc, dt):
+        """u_d - interp(vc) over dt, through the coupling inverse when the forcing is implicit
+        (ib_immersed.py:113-120 / 152-174)."""
+        ib = self.ib
+        ib.stencil(self.ib_markers, self._marker_version)
+        ib.interp(vc)
+        if self.ib_implicit:
+            return ib.implicit_lagrangian_force(self.ib_vel, dt, self._ds3)
+        return ib.lagrangian_force(self.ib_vel, dt)
+
     def _direct_force_increment(self, vc, dt):
         """Accumulated explicit increment (ib_driver_np.py:511-527)."""
         ib = self.ib
         k = self.rou * self.ib_alpha
-        ib.stencil(self.ib_markers, self._marker_version)
-        ib.interp(vc)
-        f_lag = ib.lagrangian_force(self.ib_vel, dt)
+        f_lag = self._lagrangian_force(vc, dt)
         ib.spread_into(self.ib_force_E, f_lag, self._ds3, k, accumulate=True)
         N.call("pyib_axpby", N.F64, 1.0, ptr(self.ib_force_L), float(k), ptr(f_lag), ptr(self.ib_force_L),
                C.c_int64(self.ib_force_L.numel()), dev_stream())
@@ -481,9 +489,7 @@ class HipDriver:
         """Replacement forcing at every inner iteration (ib_driver_np.py:472-489)."""
         ib = self.ib
         k = self.rou * self.ib_alpha
-        ib.stencil(self.ib_markers, self._marker_version)
-        ib.interp(vc)
-        f_lag = ib.lagrangian_force(self.ib_vel, dt)
+        f_lag = self._lagrangian_force(vc, dt)
         self.ib_force_E.zero_()
         ib.spread_into(self.ib_force_E, f_lag, self._ds3, k, accumulate=False)
         N.call("pyib_axpby", N.F64, float(k), ptr(f_lag), 0.0, ptr(None), ptr(self.ib_force_L),
@@ -497,8 +503,6 @@ class HipDriver:
         accumulated = self.ib is not None and self.ib_force_mode in {"accumulated_explicit", "fortran_accumulated"}
         if accumulated and inner < 2:
             raise ValueError("accumulated_explicit requires at least 2 fluid inner iterations")
-        if self.ib is not None and self.ib_implicit:
-            raise ValueError("implicit IB forcing is not part of the HIP path")
         self._step_count += 1
         self.time += dt
         self.helmholtz_iterations_last_step = 0

@@ -43,6 +43,8 @@ class HipImmersedBoundary:
         self.device = torch.device(device)
         self._nmark = -1
         self._version = None
+        self._stencil_serial = 0        # counts stencil rebuilds (a version may be None)
+        self._imp_key = None
 
     def _alloc(self, nmark):
         if nmark == self._nmark:
@@ -75,6 +77,7 @@ class HipImmersedBoundary:
         N.call("pyib_ib_sort", C.c_int64(nmark), ptr(self.cell), ptr(self._kt), ptr(self._vt), ptr(self.keys),
                ptr(self.vals), ptr(self._temp), C.c_int64(self._temp.numel()), dev_stream())
         self._version = version
+        self._stencil_serial += 1
 
     def interp(self, vc, markers=None, version=None, out=None):
         """Eulerian -> Lagrangian (ib_immersed.py:83-93): vel = sum vc[cell] w hh^3."""
@@ -98,6 +101,28 @@ class HipImmersedBoundary:
         self.stencil(markers, version)
         out = torch.zeros((self.ncell, 3), dtype=torch.float64, device=self.device)
         return self.spread_into(out, flag, ds3)
+
+    def implicit_lagrangian_force(self, vel_desired, dt, ds3, reg=1e-8):
+        """Implicit direct forcing (ib_immersed.py:122-174) after ``interp``: f = M^-1 (u_d - interp)/dt,
+        M = D S the marker coupling of the current stencil (pyib_ib_coupling), jittered by
+        reg * trace(M)/n on the diagonal as the reference does; M and its inverse are rebuilt
+        whenever the stencil or the marker widths change.  The dense inverse runs on the GPU."""
+        key = (self._version, self._nmark, int(ds3.data_ptr()), self._stencil_serial)
+        if self._imp_key != key or self._imp_key[0] is None:
+            n = self._nmark
+            M = torch.empty((n, n), dtype=torch.float64, device=self.device)
+            N.call("pyib_ib_coupling", C.c_int64(n), ptr(self.cell), ptr(self.w), ptr(self.keys), ptr(self.vals),
+                   ptr(ds3), self.hh ** 3, ptr(M), dev_stream())
+            M.diagonal().add_(reg * M.diagonal().sum() / n)
+            self._minv = torch.linalg.inv(M)
+            self._imp_key = key
+        rhs = self.lagrangian_force(vel_desired, dt)
+        self.f_imp = self._minv @ rhs
+        return self.f_imp
+
+    def invalidate_implicit(self):
+        """ib_immersed.py:147-150: the coupling depends on the marker positions."""
+        self._imp_key = None
 
     def lagrangian_force(self, vel_desired, dt):
         """f_L = (u_desired - interp)/dt (ib_immersed.py:113-117), after ``interp``."""

@@ -357,6 +357,12 @@ int pyib_ib_sort_noalloc(int64_t nmark, const int32_t *cell, int32_t *keys_sorte
 int pyib_ib_spread(int dtype, int64_t nmark, const int32_t *keys_sorted, const int32_t *vals_sorted,
                    const double *w, const double *flag, const double *ds3, double scale,
                    int accumulate, void *out, void *stream);
+/* Implicit forcing's marker coupling matrix (ImmersedBoundary.prepare_implicit,
+ * ib_immersed.py:122-145): M (nmark x nmark, row-major, overwritten) with
+ * M[i][j] = hh3 * sum over shared cells of w_i w_j ds3[j], from the stencil (cell, w) of
+ * pyib_ib_stencil and the cell-sorted taps of pyib_ib_sort.  Without the diagonal jitter. */
+int pyib_ib_coupling(int64_t nmark, const int32_t *cell, const double *w, const int32_t *keys_sorted,
+                     const int32_t *vals_sorted, const double *ds3, double hh3, double *M, void *stream);
 /* f_L = (vel_desired - interp)/dt (ImmersedBoundary.direct_force) */
 int pyib_ib_force(int64_t nmark, const double *vel_desired, const double *vel_ib, double dt,
                   double *f_lag, void *stream);

@@ -67,6 +67,17 @@ def config_fingerprint(config):
                           .encode("utf-8")).hexdigest()
 
 
+def rotating_sphere_velocity(markers, centre, axis, spin_ratio, radius, reference_speed):
+    """ib_case.py:566-584: rigid surface velocity omega x (x - c), |omega| = spin_ratio U / R."""
+    pts = np.asarray(markers, float)
+    d = np.asarray(axis, float)
+    n = float(np.linalg.norm(d))
+    if pts.ndim != 2 or pts.shape[1] != 3 or d.shape != (3,) or not np.isfinite(n) or n == 0.0:
+        raise ValueError("markers must be (N,3) and the rotation axis a finite non-zero three-vector")
+    omega = d / n * float(spin_ratio) * float(reference_speed) / float(radius)
+    return np.cross(omega, pts - np.asarray(centre, float))
+
+
 def sphere_center(config, t):
     """ib_case.py:587-603."""
     c = np.asarray(config["ib"]["center"], float)
@@ -172,9 +183,13 @@ def build_runtime(config, mesh=None, sparse=None, geometry=None):
         m0, v0 = motion(0.0)
         drv.set_ib(ib, m0, widths, v0, force_mode=mode)
         drv.set_motion(motion, t0=0.0)
+    elif mo["kind"] == "rotation":
+        drv.set_ib(ib, markers, widths, rotating_sphere_velocity(markers, centre, mo["axis"], float(mo["spin_ratio"]),
+                                                                 radius, uref), force_mode=mode)
     else:
         drv.set_ib(ib, markers, widths, np.zeros((count, 3)), force_mode=mode)
     drv.ib_alpha = float(ibc["alpha"])
+    drv.ib_implicit = bool(ibc["implicit"])                        # ib_case.py:766
     mt = float(mo["amplitude"]) * np.abs(np.asarray(mo["axis"], float)) if mo["kind"] == "oscillation" else np.zeros(3)
     load = ClosedSphereLoadCorrection(core.cpos[:core.ncell], 1.0 / core.vol_ci, reference_center=centre,
                                       maximum_translation=mt, radius=radius, smoothing_width=dx, density=rho,

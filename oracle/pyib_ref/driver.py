@@ -223,7 +223,7 @@ class Driver:
         return finish(m, true_res)
 
     def step(self, s, s_old, dt, dt_old, inner=3):
-        """ib_driver_np.py:412-532 (explicit IB forcing only; ``ib.implicit`` is false in every case)."""
+        """ib_driver_np.py:412-532 (explicit forcing, or implicit forcing when ``ib_implicit``)."""
         core, rou, ac, ad = self.core, self.rou, self.ac, self.ad
         accumulated = self.ib is not None and self.ib_force_mode == "accumulated_explicit"
         if accumulated and inner < 2:
@@ -244,6 +244,8 @@ class Driver:
             mk, vel = self.motion(self.time)
             self.ib_markers = np.asarray(mk, float)
             self.ib_vel = np.asarray(vel, float)
+            if getattr(self, "ib_implicit", False):
+                self.ib.invalidate_implicit()            # ib_driver_np.py:445-446
             if accumulated:
                 self.ib_force_E = self.ib.spread(self.ib_force_L, self.ib_markers, self.ib_ds)
         cg = self.cg_max_iterations
@@ -261,7 +263,8 @@ class Driver:
                 if accumulated:
                     res_c = res_c + dt * self.ib_force_E
                 else:
-                    fE, fL = self.ib.direct_force(vc[:core.ncell], self.ib_markers, self.ib_vel, dt, self.ib_ds)
+                    force = self.ib.implicit_force if getattr(self, "ib_implicit", False) else self.ib.direct_force
+                    fE, fL = force(vc[:core.ncell], self.ib_markers, self.ib_vel, dt, self.ib_ds)
                     self.ib_force_E = (self.rou * self.ib_alpha) * fE
                     self.ib_force_L = (self.rou * self.ib_alpha) * fL
                     self.ib_force_increment_E = self.ib_force_E
@@ -282,7 +285,8 @@ class Driver:
             self.helmholtz_converged_last_step = self.helmholtz_converged_last_step and self.last_helm_converged
             s, U, vc = self.calc_vel(s)
             if accumulated and i == 1:
-                dE, dL = self.ib.direct_force(vc[:core.ncell], self.ib_markers, self.ib_vel, dt, self.ib_ds)
+                force = self.ib.implicit_force if getattr(self, "ib_implicit", False) else self.ib.direct_force
+                dE, dL = force(vc[:core.ncell], self.ib_markers, self.ib_vel, dt, self.ib_ds)
                 self.ib_force_increment_E = (self.rou * self.ib_alpha) * dE
                 self.ib_force_increment_L = (self.rou * self.ib_alpha) * dL
                 self.ib_force_E = self.ib_force_E + self.ib_force_increment_E

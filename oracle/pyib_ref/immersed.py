@@ -2,8 +2,9 @@
 
 TEST INFRASTRUCTURE ONLY.  Follows ``src/pyib/solver/ib_immersed.py``:
 ``delta_h3`` 12-19, stencil build 63-81, interp 83-93, spread 95-111,
-``direct_force`` 113-120.  Spreading is a ``bincount`` over the flattened
-(marker, tap) list, i.e. a sequential sum in marker order.
+``direct_force`` 113-120, implicit forcing ``prepare_implicit`` / ``implicit_force``
+122-174.  Spreading is a ``bincount`` over the flattened (marker, tap) list, i.e. a
+sequential sum in marker order.
 """
 from __future__ import annotations
 
@@ -36,6 +37,7 @@ class ImmersedBoundary:
         self._offsets = tuple(a.ravel() for a in np.meshgrid(o, o, o, indexing="ij"))
         self._stencil = None
         self._stencil_markers = None
+        self._imp = None
 
     def stencil(self, m):
         """ib_immersed.py:46-61 value-keyed cache + 63-81 build."""
@@ -89,3 +91,42 @@ class ImmersedBoundary:
         st = self.stencil(markers)
         f_lag = (vel_desired - self.interp_stencil(vc, st)) / dt
         return self.spread_stencil(f_lag, st, ds), f_lag
+
+    def prepare_implicit(self, markers, ds, reg=1e-8):
+        """ib_immersed.py:122-145: the marker coupling matrix M = D S (D: interpolation weights
+        w hh^3, S: spreading weights w ds^3), jittered by reg * trace(M)/n on the diagonal, and its
+        dense inverse."""
+        import scipy.sparse as sp
+        cell, w = self.stencil(markers)
+        ds = np.asarray(ds, float)
+        w_int = w * self.hh ** 3
+        w_spr = w * (ds ** 3)[:, None]
+        n = cell.shape[0]
+        rows = np.repeat(np.arange(n), cell.shape[1])
+        D = sp.csr_matrix((w_int.ravel(), (rows, cell.ravel())), shape=(n, self.ncell))
+        S = sp.csr_matrix((w_spr.ravel(), (cell.ravel(), rows)), shape=(self.ncell, n))
+        M = np.asarray((D @ S).todense())
+        M += reg * np.trace(M) / n * np.eye(n)
+        self._imp = dict(cell=cell, w_int=w_int, w_spr=w_spr, M=M, Minv=np.linalg.inv(M), n=n, ds=ds.copy())
+        return self
+
+    def invalidate_implicit(self):
+        """ib_immersed.py:147-150."""
+        self._imp = None
+
+    def implicit_force(self, vc, markers, vel_desired, dt, ds):
+        """ib_immersed.py:152-174: f = M^-1 (u_desired - interp(vc))/dt, F_E = spread(f)."""
+        self.stencil(markers)
+        ds = np.asarray(ds, float)
+        if self._imp is None or self._imp["n"] != markers.shape[0] or not np.array_equal(ds, self._imp["ds"]):
+            self.prepare_implicit(markers, ds)
+        c = self._imp
+        cell = c["cell"]
+        vel_ib = np.empty((cell.shape[0], vc.shape[1]))
+        for k in range(vc.shape[1]):
+            vel_ib[:, k] = (vc[cell, k] * c["w_int"]).sum(axis=1)
+        f = c["Minv"] @ ((np.asarray(vel_desired, float) - vel_ib) / dt)
+        fE = np.zeros((self.ncell, vc.shape[1]))
+        for k in range(vc.shape[1]):
+            fE[:, k] = np.bincount(cell.ravel(), (f[:, k:k + 1] * c["w_spr"]).ravel(), minlength=self.ncell)
+        return fE, f
