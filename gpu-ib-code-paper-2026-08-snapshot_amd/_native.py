@@ -37,7 +37,8 @@ IB_NOALLOC_MAX_MARKERS = 16384     # PYIB_IB_NOALLOC_MAX_MARKERS of include/pyib
 
 class Level(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("reserved", C.c_int32),
-                ("cell_off", C.c_int64), ("e_off", C.c_int64 * 3), ("h", C.c_double)]
+                ("cell_off", C.c_int64), ("e_off", C.c_int64 * 3), ("h", C.c_double),
+                ("hole_lo", C.c_int32 * 3), ("hole_hi", C.c_int32 * 3)]
 
 
 class Grid(C.Structure):
@@ -49,6 +50,8 @@ class Grid(C.Structure):
         ("v_nslice", C.c_int64), ("v_row", vp), ("v_width", vp), ("v_sptr", vp), ("v_col", vp), ("v_coef", vp),
         ("v_rw", vp),
         ("g3_nslice", C.c_int64), ("g3_row", vp), ("g3_width", vp), ("g3_sptr", vp), ("g3_col", vp), ("g3_coef", vp),
+        ("edge_cls", vp), ("cls_prec", vp), ("prec_level", C.c_double * MAX_LEVELS),
+        ("prec_level_g3", C.c_double * MAX_LEVELS),
     ]
 
 

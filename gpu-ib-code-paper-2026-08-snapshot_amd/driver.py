@@ -111,6 +111,7 @@ class HipDriver:
             pi[self.grid.lay.edge_map] = self.prec_host
             self.prec_i = core.tensor(pi)
             self.prec_i32 = self.prec_i.float()
+            self.grid.set_preconditioner(pi, {"float64": self.prec_i, "float32": self.prec_i32})
             if hasattr(self, "_cgi"):
                 self._cgi.prec = ptr(self.prec_i)
                 if hasattr(self, "_cgi32"):

@@ -16,8 +16,8 @@ import torch
 
 from . import _native as N
 from .core import dev_stream, ptr
-from .layout import (MAX_LEVELS, build_layout, composite_rows, dir_mask_internal, g1v_flags, row_volumes,
-                     to_sell)
+from .layout import (MAX_LEVELS, build_layout, composite_rows, dir_mask_internal, edge_classes, g1v_flags,
+                     row_volumes, to_sell)
 
 
 class HipGrid:
@@ -84,6 +84,8 @@ class HipGrid:
             L.cell_off = lv.cell_off
             L.e_off[0], L.e_off[1], L.e_off[2] = lv.e_off
             L.h = lv.h
+            for a in range(3):
+                L.hole_lo[a], L.hole_hi[a] = int(lv.hole_lo[a]), int(lv.hole_hi[a])
         g.cell_reg, g.cell_reg_g1 = ptr(self._cell_reg), ptr(self._cell_reg_g1)
         g.edge_reg, g.dir = ptr(self._edge_reg), ptr(self._dir)
         g.cell_g1v = ptr(self._cell_g1v)
@@ -97,6 +99,11 @@ class HipGrid:
             setattr(g, pre + "_col", ptr(cc["col"]))
             setattr(g, pre + "_coef", ptr(k[name]))
         g.v_rw = ptr(c["vis"]["rw"])
+        cls = getattr(self, "_cls", None)
+        if cls is not None and prec in cls["prec"]:
+            g.edge_cls, g.cls_prec = ptr(self._edge_cls), ptr(cls["prec"][prec])
+            for i in range(MAX_LEVELS):
+                g.prec_level[i], g.prec_level_g3[i] = float(cls["plev"][i]), float(cls["pg3"][i])
         return g
 
     def set_dir_mask(self, dir_mask_ref):
@@ -104,12 +111,28 @@ class HipGrid:
         zero (w = 0, r unchanged at 0), so the G3 composite keeps only the other rows."""
         m = dir_mask_internal(self.lay, dir_mask_ref)
         self._dir.copy_(torch.from_numpy(m))
+        self._dir_host = m
+        self._cls = None            # the preconditioner classes follow the mask: set them again
         g3 = self.lay.g3
         self._comp["g3"] = self._sell(composite_rows(g3, np.flatnonzero(m[np.asarray(g3.rows)] == 0)))
         for prec in self._struct:
             dt = torch.float64 if prec == "float64" else torch.float32
             self._coef[prec]["g3"] = torch.from_numpy(np.ascontiguousarray(self._comp["g3"]["coef64"])).to(
                 self.device, dt)
+            self._struct[prec] = self._make(prec)
+
+    def set_preconditioner(self, prec_int_host, prec_tensors):
+        """Classes of the Jacobi preconditioner (``layout.edge_classes``) for the PCG iteration.
+        ``prec_int_host``: the float64 inverse diagonal in internal numbering; ``prec_tensors``:
+        {precision: the device copy a pyib_cg of that precision points at}.  The iteration uses the
+        classes only with those arrays; with any other it reads its prec as is."""
+        dir_host = getattr(self, "_dir_host", None)
+        if dir_host is None:
+            raise RuntimeError("set_dir_mask comes before set_preconditioner")
+        cls, plev, pg3 = edge_classes(self.lay, dir_host, prec_int_host)
+        self._edge_cls = self._t(cls, torch.uint8)
+        self._cls = {"plev": plev, "pg3": pg3, "prec": dict(prec_tensors)}
+        for prec in self._struct:
             self._struct[prec] = self._make(prec)
 
     def struct(self, prec="float64"):

@@ -443,15 +443,19 @@ def composite_rows(comp, sel):
 def g1v_flags(lay):
     """Per internal cell, the flags of the fused G1+V pass: bit 0 = regular G1 row (x =
     interp curl p formed by the stencil), bit 1 = that x must also be stored, because a
-    composite V row reads it (as its own row or a column), bit 2 = regular V row.  The pass
-    forms x in LDS and stores only the bit-1 cells."""
+    composite V row reads it (as its own row or a column), bit 2 = regular V row, bit 3 = x
+    comes from memory (a real cell whose G1 row is composite: the composite G1 pass wrote it).
+    The pass forms x in LDS, stores only the bit-1 cells, and takes x = 0 on the cells with
+    neither bit 0 nor bit 3 (the hole, whose x no stencil row reads)."""
     keep = np.zeros(lay.ncell, bool)
     keep[np.asarray(lay.vis.rows, dtype=np.int64)] = True
     keep[np.asarray(lay.vis.col, dtype=np.int64)] = True
     reg = lay.reg_g1.astype(bool)
     regv = lay.reg_cell.astype(bool)
+    real = np.zeros(lay.ncell, bool)
+    real[np.asarray(lay.cell_map, dtype=np.int64)] = True
     return (reg.astype(np.uint8) | ((reg & keep).astype(np.uint8) << 1)
-            | (regv.astype(np.uint8) << 2)).astype(np.uint8)
+            | (regv.astype(np.uint8) << 2) | ((real & ~reg).astype(np.uint8) << 3)).astype(np.uint8)
 
 
 def row_volumes(lay, geo, rows):
@@ -487,6 +491,47 @@ def dir_mask_internal(lay, dir_mask_ref):
     m = np.ones(lay.nedge, np.uint8)
     m[lay.edge_map] = np.asarray(dir_mask_ref, bool).astype(np.uint8)
     return m
+
+
+def edge_classes(lay, dir_int, prec_int):
+    """Classes of the Jacobi preconditioner over the internal edge slots.
+
+    ``prec_int`` is the inverse diagonal of ``diag_preconditioner`` (ib_driver_np.py:78-88) in
+    internal numbering (0 on Dirichlet rows and empty slots).  On the edges of a level whose four
+    faces are whole h x h squares it is the same number (the sum of 1/h over the four faces,
+    inverted), so the PCG iteration can take it from a per-level table.  The table value of a
+    level is its most frequent value over the live slots, and an edge is given bit 2 only where
+    its own value equals it bitwise, so the table never changes a result.
+
+    Returns (cls uint8 (nedge,), prec_level (MAX_LEVELS,), prec_level_g3 (MAX_LEVELS,)):
+    cls bit 0 = regular G3 row, bit 1 = live (neither empty nor Dirichlet), bit 2 = prec equals
+    prec_level of its level, bits 4-6 = level; prec_level_g3[l] = prec_level[l] when every
+    regular G3 row of level l has bit 2, else 0."""
+    prec_int = np.asarray(prec_int, np.float64)
+    dir_int = np.asarray(dir_int)
+    if prec_int.shape != (lay.nedge,) or dir_int.shape != (lay.nedge,):
+        raise ValueError("edge_classes needs the internal Dirichlet mask and preconditioner")
+    reg = lay.reg_edge.astype(bool)
+    live = dir_int == 0
+    cls = reg.astype(np.uint8) | (live.astype(np.uint8) << 1)
+    plev = np.zeros(MAX_LEVELS)
+    pg3 = np.zeros(MAX_LEVELS)
+    bounds = [lv.e_off[0] for lv in lay.levels] + [lay.nedge]
+    for L in range(len(lay.levels)):
+        a, b = bounds[L], bounds[L + 1]
+        cls[a:b] |= np.uint8(L << 4)
+        seg, lv_live = prec_int[a:b], live[a:b]
+        if not lv_live.any():
+            continue
+        vals, counts = np.unique(seg[lv_live], return_counts=True)
+        c = float(vals[np.argmax(counts)])
+        closed = lv_live & (seg == c)
+        cls[a:b] |= closed.astype(np.uint8) << 2
+        plev[L] = c
+        r = reg[a:b]
+        if c != 0.0 and bool(np.all(closed[r])):
+            pg3[L] = c
+    return cls, plev, pg3
 
 
 def to_internal_edges(lay, x_ref):

@@ -96,6 +96,8 @@ typedef struct pyib_level {
   int64_t cell_off;         /* first internal cell of the level */
   int64_t e_off[3];         /* first internal x/y/z edge of the level */
   double h;                 /* spacing */
+  int32_t hole_lo[3], hole_hi[3];   /* cells [hole_lo, hole_hi) of the box belong to the finer
+                                     * level (no real cell, no real edge; empty: lo == hi) */
 } pyib_level;
 
 typedef struct pyib_grid {
@@ -111,7 +113,8 @@ typedef struct pyib_grid {
   const uint8_t *dir;       /* nedge: Dirichlet edge or empty slot (residual forced to 0) */
   const uint8_t *cell_g1v;  /* ncell, flags of the fused G1+V pass: bit 0 = cell_reg_g1, bit 1 = its
                              * x = interp(curl p) is stored for the composite V rows (the pass keeps
-                             * the rest in LDS), bit 2 = cell_reg */
+                             * the rest in LDS), bit 2 = cell_reg, bit 3 = x is read from memory
+                             * (a real cell with a composite G1 row); x = 0 on the other cells */
   /* composite rows of the other (interface/boundary) rows in sliced ELL: slices of 64
    * rows (one wavefront), entry j of lane l of slice s at sptr[s] + 64*j + l, width[s]
    * entries per lane, row[64*s + l] = internal row id or -1 (padding, col 0, coef 0) */
@@ -122,6 +125,17 @@ typedef struct pyib_grid {
   const double *v_rw;   /* 64*v_nslice: volume (1/vol_ci) of the row's cell, 0 on padding */
   int64_t g3_nslice; const int32_t *g3_row, *g3_width; const int64_t *g3_sptr;
   const int32_t *g3_col; const void *g3_coef;   /* coef entries x 3 */
+  /* Classes of the Jacobi preconditioner over the edge slots.  Used by the PCG iteration when
+   * edge_cls is non-NULL and the pyib_cg's prec is cls_prec; any other prec array is read as is.
+   * On a regular edge of a level the inverse diagonal is the same number at every edge, so the
+   * iteration takes it from prec_level instead of reading prec, and it skips the slots whose
+   * direction stays zero (empty slots and Dirichlet rows: r = 0 and p = 0 there). */
+  const uint8_t *edge_cls;  /* nedge: bit 0 = edge_reg, bit 1 live (not empty, not Dirichlet),
+                             * bit 2 prec == prec_level[level] bitwise, bits 4-6 the level */
+  const void *cls_prec;     /* the prec array (of this dtype) the classes were derived from */
+  double prec_level[PYIB_MAX_LEVELS];     /* prec on the bit-2 slots of each level */
+  double prec_level_g3[PYIB_MAX_LEVELS];  /* prec_level when every regular G3 row of the level is
+                                           * bit 2, else 0 (then the G3 pass reads prec) */
 } pyib_grid;
 
 const char *pyib_last_error(void);

@@ -122,6 +122,51 @@ def test_helm_solve(mesh3, cavity, structured):
     assert abs(hd.last_helm_tolerance_ratio - r_mf) <= max(1e-6 * r_mf, 10.0 * abs(r_csr - r_mf))
 
 
+@pytest.mark.parametrize("cavity", [False, True])
+def test_grid_pcg_preconditioner_classes_change_no_bit(mesh3, cavity):
+    """The PCG iteration with the preconditioner classes (prec from the per-level table, empty
+    slots and Dirichlet rows skipped) against the same iterations reading the whole prec array:
+    x, r, p and every CG scalar equal bit for bit, in float64 and float32, across the plain and
+    the skip-last iteration blocks."""
+    import ctypes as C
+    from pyib_hip import _native as N
+    from pyib_hip.core import dev_stream, ptr
+    hc = HipCore(mesh3)
+    hd = HipDriver(hc, mesh3, visc=0.05, cg_max_iterations=3000, structured=True, precision="mixed")
+    if cavity:
+        hd.set_cavity(U_lid=0.0, spanwise_zerograd=False)
+    g = hd.grid
+    for prec in ("float64", "float32"):
+        st = g._struct[prec]
+        assert st.edge_cls is not None and st.cls_prec is not None
+        assert all(st.prec_level_g3[i] != 0.0 for i in range(st.nlevels))
+    rng = np.random.default_rng(11)
+    res = rng.standard_normal(hc.nedge)
+    res[hd.dir_mask] = 0.0
+    s0 = rng.standard_normal(hc.nedge) * 1e-2
+    hdt = 0.5 * 0.0043633
+    for prec, dt, pr in (("float64", torch.float64, hd.prec_i), ("float32", torch.float32, hd.prec_i32)):
+        out = []
+        for use_cls in (True, False):
+            vec = {k: torch.zeros(g.nedge, dtype=dt, device="cuda") for k in "xrpw"}
+            pr_use = pr if use_cls else pr.clone()
+            scal = torch.zeros(N.NSCAL, dtype=torch.float64, device="cuda")
+            cg = N.CG(ptr(vec["x"]), ptr(vec["r"]), ptr(vec["p"]), ptr(vec["w"]), ptr(pr_use), ptr(scal))
+            r_i, s_i = g.to_internal(T(res), dtype=dt), g.to_internal(T(s0), dtype=dt)
+            m, ws = g.struct(prec), g.work_struct(prec)
+            N.call("pyib_grid_cg_begin", m, ws, C.byref(cg), ptr(r_i), ptr(s_i), 1e-10, 1e-9, dev_stream())
+            N.call("pyib_grid_cg_iterate", m, ws, C.byref(cg), 1.0, 0.05, hdt, 37, 0, dev_stream())
+            N.call("pyib_grid_cg_iterate", m, ws, C.byref(cg), 1.0, 0.05, hdt, 3, 1, dev_stream())
+            N.call("pyib_grid_cg_direction", m, C.byref(cg), dev_stream())
+            N.call("pyib_grid_cg_iterate", m, ws, C.byref(cg), 1.0, 0.05, hdt, 5, 0, dev_stream())
+            torch.cuda.synchronize()
+            out.append(([H(vec[k]) for k in "xrp"], H(scal)))
+        for a, b in zip(out[0][0], out[1][0]):
+            assert np.isfinite(a).all() and np.array_equal(a, b)
+        assert np.array_equal(out[0][1], out[1][1])
+        print(f"[gpu] prec classes cavity={cavity} {prec}: 45 iterations bitwise equal, eta {out[0][1][0]:.6e}")
+
+
 @pytest.mark.parametrize("h,boxes", [(0.0625, "nested3"), (0.5 / 24, "nested2"), (0.125, "wake")])
 def test_grid_flow_mat(h, boxes):
     """Structured stencils + composite rows == oracle flow_mat through the signed renumbering."""

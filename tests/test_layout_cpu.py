@@ -304,3 +304,80 @@ def test_composite_rows_keeps_each_selected_row():
         assert np.array_equal(sub.col[sub.ptr[i]:sub.ptr[i + 1]], c.col[a:b])
         assert np.array_equal(sub.coef[sub.ptr[i]:sub.ptr[i + 1]], c.coef[a:b])
     assert int(sub.ptr[-1]) == int(sum(c.ptr[r + 1] - c.ptr[r] for r in sel))
+
+
+@pytest.mark.parametrize("levels", [2, 3])
+def test_preconditioner_classes(levels):
+    """The per-level table of the Jacobi preconditioner: bit 2 only where an edge's own inverse
+    diagonal equals the table bitwise; the table is the closed form h/4 (four whole h x h faces,
+    1/h each); every regular G3 row carries it; live = neither an empty slot nor Dirichlet."""
+    from pyib_hip.geometry import diag_preconditioner
+    from pyib_hip.layout import MAX_LEVELS, dir_mask_internal, edge_classes
+    m = PM.make_nested_mesh(0.0625, PM.nested_box_boxes([1.5, 1.25, 1.25], levels))
+    g = Geometry(m)
+    lay = build_layout(m, g, build_tables(g))
+    dm = np.asarray(m.bc_edge) > 0
+    pint = np.zeros(lay.nedge)
+    pint[lay.edge_map] = diag_preconditioner(g, dm)
+    dint = dir_mask_internal(lay, dm)
+    cls, plev, pg3 = edge_classes(lay, dint, pint)
+    assert cls.dtype == np.uint8 and plev.shape == pg3.shape == (MAX_LEVELS,)
+    live = (cls & 2) > 0
+    assert np.array_equal(live, dint == 0)
+    assert not live[lay.edge_inv < 0].any() and not live[lay.edge_map[dm]].any()
+    assert np.array_equal(cls & 1, lay.reg_edge)
+    lvl = (cls >> 4) & 7
+    bounds = [lv.e_off[0] for lv in lay.levels] + [lay.nedge]
+    closed = (cls & 4) > 0
+    assert not closed[~live].any()
+    for L, lv in enumerate(lay.levels):
+        assert np.all(lvl[bounds[L]:bounds[L + 1]] == L)
+        here = closed & (lvl == L)
+        assert np.all(pint[here] == plev[L])
+        assert plev[L] == pytest.approx(lv.h / 4.0, rel=1e-14, abs=0)
+        assert pg3[L] == plev[L]
+        reg = (cls & 1).astype(bool) & (lvl == L)
+        assert np.all(closed[reg])
+        # the slots left to read their prec are the live edges whose value differs
+        rest = live & ~closed & (lvl == L)
+        assert np.all(pint[rest] != plev[L])
+        assert here.sum() > 0.8 * (live & (lvl == L)).sum()
+    # a planted prec value on one regular edge takes bit 2 away from it and the G3 table off
+    e = int(np.flatnonzero((cls & 1) & closed)[0])
+    bad = pint.copy()
+    bad[e] = np.nextafter(bad[e], 1.0)
+    cls2, plev2, pg32 = edge_classes(lay, dint, bad)
+    L = int((cls2[e] >> 4) & 7)
+    assert not (cls2[e] & 4) and pg32[L] == 0.0 and plev2[L] == plev[L]
+
+
+def test_hole_cells_own_no_regular_row():
+    """The tiled kernels skip a tile whose cells all lie in the level's hole: none of those cells
+    is a regular G1 or V row or has any flag of the fused pass, and none of the x-, y- and
+    z-edges (i, j, k) they own is a regular G3 row or a real edge."""
+    m = PM.make_nested_mesh(0.0625, PM.nested_box_boxes([1.5, 1.25, 1.25], 3))
+    g = Geometry(m)
+    lay = build_layout(m, g, build_tables(g))
+    flags = g1v_flags(lay)
+    real_e = np.zeros(lay.nedge, bool)
+    real_e[lay.edge_map] = True
+    holes = 0
+    for lv in lay.levels:
+        a, b = lv.hole_lo, lv.hole_hi
+        if not b[0] > a[0]:
+            continue
+        holes += 1
+        cells = np.zeros((lv.nx, lv.ny, lv.nz), bool)
+        cells[a[0]:b[0], a[1]:b[1], a[2]:b[2]] = True
+        sl = slice(lv.cell_off, lv.cell_off + lv.ncell)
+        assert not lay.reg_cell[sl][cells.ravel()].any() and not lay.reg_g1[sl][cells.ravel()].any()
+        assert not flags[sl][cells.ravel()].any()
+        for d, blk in enumerate(blocks(lay, lv, np.arange(lay.nedge))):
+            own = blk[:lv.nx, :lv.ny, :lv.nz][cells]
+            assert not lay.reg_edge[own].any() and not real_e[own].any()
+    assert holes == 2
+    # bit 3 (x read from memory) marks exactly the real cells whose G1 row is composite
+    real = np.zeros(lay.ncell, bool)
+    real[lay.cell_map] = True
+    assert np.array_equal((flags & 8) > 0, real & (lay.reg_g1 == 0))
+    assert set(np.flatnonzero((flags & 8) > 0)) <= set(np.asarray(lay.g1.rows).tolist())
