@@ -139,6 +139,11 @@ _SIGS = {
     "pyib_mesh_build": (C.c_int, [C.c_int32, vp, vp, vp, vp, C.POINTER(vp), vp]),
     "pyib_mesh_fetch": (C.c_int, [vp] + [vp] * 15),
     "pyib_mesh_free": (C.c_int, [vp]),
+    "pyib_layout_build": (C.c_int, [C.c_int32, vp, vp, vp, C.c_int64, C.c_int64, C.c_int64, vp, C.c_int64, vp, vp,
+                                    C.c_int64, C.c_int64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                    vp, C.c_int32, C.c_double, C.c_int32, C.POINTER(vp), vp]),
+    "pyib_layout_fetch": (C.c_int, [vp] + [vp] * 20),
+    "pyib_layout_free": (C.c_int, [vp]),
     "pyib_interior_momentum_adjoint": (C.c_int, [C.c_int64, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                                  C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                                  C.c_double, vp, vp, vp, vp]),

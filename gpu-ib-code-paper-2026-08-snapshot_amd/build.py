@@ -7,7 +7,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = [os.path.join(HERE, "csrc", n)
-       for n in ("pyib_hip.hip", "grid.hip", "adjoint.hip", "geometry.cpp", "mesh.cpp",
+       for n in ("pyib_hip.hip", "grid.hip", "adjoint.hip", "geometry.cpp", "mesh.cpp", "layout.cpp",
                  "pressure.hip", "implicit.hip")]
 OUT = os.path.join(HERE, "libpyib_hip.so")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",

@@ -361,7 +361,7 @@ def layout_fingerprint(mesh, markers, widths):
 
 _SOURCE_FILES = ("case.py", "core.py", "driver.py", "geometry.py", "grid.py", "immersed.py", "interior.py",
                  "kinematics.py", "layout.py", "mesh.py", "tables.py", "csrc/pyib_hip.hip", "csrc/grid.hip",
-                 "csrc/geometry.cpp", "csrc/mesh.cpp")
+                 "csrc/geometry.cpp", "csrc/mesh.cpp", "csrc/layout.cpp", "csrc/implicit.hip")
 
 
 def source_hashes():

@@ -86,29 +86,24 @@ def _cells_of_level(lv):
     return m
 
 
-def build_layout(mesh, geo, tables, margin=1, tol=1e-13, demote_neighbours=False):
-    """Internal layout + classification.  ``tol`` is the relative tolerance of the
-    closed-form coefficient check (0: bitwise).  The default keeps rows whose
-    geometry differs from the closed form only by the rounding of non-dyadic
-    node coordinates (~1e-16 relative).  ``margin`` cells next to the box and the
-    hole are never regular (the geometry check demotes the first layer anyway; one
-    cell keeps every stencil read inside its level block).  ``demote_neighbours``
-    also demotes the neighbours of geometrically irregular cells (not needed for
-    exactness: the V stencil reads its neighbours' values, whichever pass wrote them)."""
+def _levels(mesh):
+    """Level boxes and the internal numbering's offsets (cells, then the x/y/z edge blocks of
+    each level in turn)."""
     N = mesh.n_layers
     if N > MAX_LEVELS:
         raise ValueError(f"at most {MAX_LEVELS} levels")
     levels = []
     coff, eoff = 0, 0
     for L in range(N):
-        nx, ny, nz = mesh.layer_shape[L]
+        nx, ny, nz = (int(v) for v in mesh.layer_shape[L])
         if L > 0:
             hl = tuple(int(v) for v in mesh.layer_hole_lo[L])
-            hs = tuple(v // 2 for v in mesh.layer_shape[L - 1])
+            hs = tuple(int(v) // 2 for v in mesh.layer_shape[L - 1])
             hh = tuple(hl[a] + hs[a] for a in range(3))
         else:
             hl, hh = (0, 0, 0), (0, 0, 0)
-        lv = Level(nx, ny, nz, float(mesh.layer_spacing[L]), tuple(mesh.layer_min[L]), hl, hh, coff, (0, 0, 0))
+        lv = Level(nx, ny, nz, float(mesh.layer_spacing[L]), tuple(float(v) for v in mesh.layer_min[L]), hl, hh,
+                   coff, (0, 0, 0))
         lv.ncell = nx * ny * nz
         offs = []
         for d in range(3):
@@ -117,9 +112,73 @@ def build_layout(mesh, geo, tables, margin=1, tol=1e-13, demote_neighbours=False
         lv.e_off = tuple(offs)
         coff += lv.ncell
         levels.append(lv)
-    ncell_i, nedge_i = coff, eoff
-    if max(ncell_i, nedge_i) >= 2 ** 31 - 1:
+    if max(coff, eoff) >= 2 ** 31 - 1:
         raise ValueError("mesh too large for int32 internal indices")
+    return levels, coff, eoff
+
+
+def build_layout(mesh, geo, tables, margin=1, tol=1e-13, demote_neighbours=False):
+    """Internal layout + classification + composite rows, built in C++ (``pyib_layout_build`` in
+    ``csrc/layout.cpp``); array for array equal to ``build_layout_numpy`` (the same arguments)."""
+    import ctypes as C
+    from . import _native as N
+    levels, ncell_i, nedge_i = _levels(mesh)
+    li = np.array([[lv.nx, lv.ny, lv.nz, *lv.hole_lo, *lv.hole_hi, lv.cell_off, *lv.e_off] for lv in levels],
+                  np.int64)
+    lf = np.array([[lv.h, *lv.lo] for lv in levels], np.float64)
+    c64 = lambda a: np.ascontiguousarray(a, np.float64)   # noqa: E731
+    i64 = lambda a: np.ascontiguousarray(a, np.int64)     # noqa: E731
+    i32 = lambda a: np.ascontiguousarray(a, np.int32)     # noqa: E731
+    keep = [li, lf, i64(mesh.layer_cell_counts), c64(mesh.nodes), i64(mesh.edges_to_nodes), c64(geo.te),
+            c64(geo.vol_ci), i64(geo.F2C), c64(geo.nf), c64(geo.rf1), c64(geo.rf2), c64(geo.AoL), c64(geo.Af),
+            i32(tables.curl_ptr), i32(tables.curl_idx), i32(tables.rot_ptr), i32(tables.rot_idx),
+            i32(tables.cf_ptr), i32(tables.cf_mid), i32(tables.cf_face), i32(tables.cf_other)]
+    p = lambda a: a.ctypes.data_as(C.c_void_p)            # noqa: E731
+    (li, lf, lcc, nodes, e2n, te, vol, f2c, nf, rf1, rf2, aol, af, cp, ci, rp, ri, fp, fm, ff, fo) = keep
+    handle = C.c_void_p()
+    counts = np.zeros(9, np.int64)
+    N.call("pyib_layout_build", C.c_int32(len(levels)), p(li), p(lf), p(lcc), C.c_int64(ncell_i),
+           C.c_int64(nedge_i), C.c_int64(nodes.shape[0]), p(nodes), C.c_int64(e2n.shape[0]), p(e2n), p(te),
+           C.c_int64(geo.ncell), C.c_int64(geo.nface), p(vol), p(f2c), p(nf), p(rf1), p(rf2), p(aol), p(af),
+           p(cp), p(ci), p(rp), p(ri), p(fp), p(fm), p(ff), p(fo), C.c_int32(margin), C.c_double(tol),
+           C.c_int32(int(bool(demote_neighbours))), C.byref(handle), p(counts))
+    try:
+        nref_e = int(e2n.shape[0])
+        cell_map, edge_map = np.empty(geo.ncell, np.int64), np.empty(nref_e, np.int64)
+        edge_sigma, edge_inv = np.empty(nref_e, np.int8), np.empty(nedge_i, np.int64)
+        sigma_int = np.empty(nedge_i, np.int8)
+        reg_cell, reg_edge, reg_g1 = np.empty(ncell_i, np.uint8), np.empty(nedge_i, np.uint8), np.empty(ncell_i, np.uint8)
+        comps = []
+        for k, ncomp in enumerate((3, 1, 3)):
+            nr, nnz = int(counts[2 * k]), int(counts[2 * k + 1])
+            coef = np.empty((nnz, 3) if ncomp == 3 else (nnz,), np.float64)
+            comps.append(Composite(np.empty(nr, np.int32), np.empty(nr + 1, np.int32), np.empty(nnz, np.int32), coef))
+        arrs = [cell_map, edge_map, edge_sigma, edge_inv, sigma_int, reg_cell, reg_edge, reg_g1]
+        for c in comps:
+            arrs += [c.rows, c.ptr, c.col, c.coef]
+        N.call("pyib_layout_fetch", handle, *[p(a) for a in arrs])
+    finally:
+        N.call("pyib_layout_free", handle)
+    lay = StructuredLayout(levels, ncell_i, nedge_i, cell_map, edge_map, edge_sigma, edge_inv, sigma_int,
+                           reg_cell, reg_edge, reg_g1, comps[0], comps[1], comps[2])
+    lay.stats.update(regular_cells=int(counts[6]), regular_g1_cells=int(counts[7]), regular_edges=int(counts[8]))
+    lay.stats.update(irregular_cells=int(counts[2]), irregular_g1_cells=int(counts[0]), irregular_edges=int(counts[4]),
+                     g1_nnz=int(counts[1]), v_nnz=int(counts[3]), g3_nnz=int(counts[5]))
+    return lay
+
+
+def build_layout_numpy(mesh, geo, tables, margin=1, tol=1e-13, demote_neighbours=False):
+    """NumPy statement of ``build_layout`` (the construction the C++ builder follows; the CPU
+    tests compare the two array for array).  ``tol`` is the relative tolerance of the
+    closed-form coefficient check (0: bitwise).  The default keeps rows whose
+    geometry differs from the closed form only by the rounding of non-dyadic
+    node coordinates (~1e-16 relative).  ``margin`` cells next to the box and the
+    hole are never regular (the geometry check demotes the first layer anyway; one
+    cell keeps every stencil read inside its level block).  ``demote_neighbours``
+    also demotes the neighbours of geometrically irregular cells (not needed for
+    exactness: the V stencil reads its neighbours' values, whichever pass wrote them)."""
+    levels, ncell_i, nedge_i = _levels(mesh)
+    N = len(levels)
     # ---- cells: reference layer order = C order of the live cells of each level
     cell_map = np.empty(geo.ncell, np.int64)
     start = 0

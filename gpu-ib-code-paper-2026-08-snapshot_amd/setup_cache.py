@@ -18,7 +18,7 @@ from pathlib import Path
 import numpy as np
 
 _SOURCES = ("mesh.py", "geometry.py", "tables.py", "layout.py", "setup_cache.py", "csrc/geometry.cpp",
-            "csrc/mesh.cpp")
+            "csrc/mesh.cpp", "csrc/layout.cpp")
 
 
 def cache_dir():

@@ -258,6 +258,34 @@ int pyib_mesh_fetch(void *handle, double *nodes, int64_t *cells, int64_t *bfaces
                     int64_t *layer_nodes);
 int pyib_mesh_free(void *handle);
 
+/* ---- structured layout of the elliptic operator on the host (setup; our design, no
+ * counterpart in the reference: it renumbers the mesh of make_euler_mesh.py:330-463 into
+ * per-level dense boxes and assembles, from the IBCore geometry of ib_core_np.py:24-126, the
+ * rows of A = rot.integr.V.interp.curl (ib_core_np.py:168-175) that are not constant stencils).
+ * lev_i64 (nlevels,13): nx, ny, nz, hole lo (3), hole hi (3), cell offset, x/y/z edge-block
+ * offsets; lev_f64 (nlevels,4): h, box corner (3); layer_cells (nlevels).  The mesh arrays are
+ * nodes (nnode,3), edges_to_nodes (nedge,2), tangents te (nedge,3); the geometry vol_ci (ncell),
+ * faces_to_cells (nface,2), nf/rf1/rf2 (nface,3), AoL/Af (nface); the gather tables those of
+ * pyib_tables.  counts[0..8] = G1 rows, G1 entries, V rows, V entries, G3 rows, G3 entries,
+ * regular V cells, regular G1 cells, regular G3 edges.  pyib_layout_fetch copies out cell_map
+ * (ncell), edge_map/edge_sigma (nedge), edge_inv/sigma_int/reg_edge (internal edges),
+ * reg_cell/reg_g1 (internal cells), then rows/ptr/col/coef of G1 (coef x3), V (x1), G3 (x3);
+ * a NULL output is skipped.  pyib_layout_free releases the handle. */
+int pyib_layout_build(int32_t nlevels, const int64_t *lev_i64, const double *lev_f64, const int64_t *layer_cells,
+                      int64_t ncell_i, int64_t nedge_i, int64_t nnode, const double *nodes, int64_t nedge,
+                      const int64_t *e2n, const double *te, int64_t ncell, int64_t nface, const double *vol_ci,
+                      const int64_t *f2c, const double *nf, const double *rf1, const double *rf2, const double *aol,
+                      const double *af, const int32_t *curl_ptr, const int32_t *curl_idx, const int32_t *rot_ptr,
+                      const int32_t *rot_idx, const int32_t *cf_ptr, const int32_t *cf_mid, const int32_t *cf_face,
+                      const int32_t *cf_other, int32_t margin, double tol, int32_t demote_neighbours,
+                      void **handle, int64_t *counts);
+int pyib_layout_fetch(void *handle, int64_t *cell_map, int64_t *edge_map, int8_t *edge_sigma, int64_t *edge_inv,
+                      int8_t *sigma_int, uint8_t *reg_cell, uint8_t *reg_edge, uint8_t *reg_g1, int32_t *g1_rows,
+                      int32_t *g1_ptr, int32_t *g1_col, double *g1_coef, int32_t *v_rows, int32_t *v_ptr,
+                      int32_t *v_col, double *v_coef, int32_t *g3_rows, int32_t *g3_ptr, int32_t *g3_col,
+                      double *g3_coef);
+int pyib_layout_free(void *handle);
+
 /* ---- the same CG on the structured layout (ib_driver_np.py:311-410) ----
  * Workspace cell_a/cell_b hold 3*ncell (internal) values; partials >= 4*PYIB_MAX_PARTIALS.
  * flow_mat: w = A p (0 on dir; a caller that drops the Dirichlet rows from the G3 composite

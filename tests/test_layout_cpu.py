@@ -381,3 +381,53 @@ def test_hole_cells_own_no_regular_row():
     real[lay.cell_map] = True
     assert np.array_equal((flags & 8) > 0, real & (lay.reg_g1 == 0))
     assert set(np.flatnonzero((flags & 8) > 0)) <= set(np.asarray(lay.g1.rows).tolist())
+
+
+@pytest.mark.parametrize("h, boxes, kw", [
+    (0.0625, "nested3", {}), (0.5 / 24, "nested2", {}), (0.125, "wake", {}), (0.125, "concentric", {}),
+    (0.0625, "nested3", {"tol": 0.0}), (0.0625, "nested3", {"margin": 2, "demote_neighbours": True}),
+])
+def test_native_layout_equals_the_numpy_statement(h, boxes, kw):
+    """The C++ builder (pyib_layout_build) produces every array of the NumPy statement
+    bit for bit: numbering, orientation, per-pass regularity, and the composite rows with
+    their duplicate entries summed in NumPy's add.reduceat order."""
+    from pyib_hip.layout import build_layout_numpy
+    bx = {"nested3": PM.nested_box_boxes([1.5, 1.25, 1.25], 3), "nested2": PM.nested_box_boxes([1.0, 0.75, 0.75], 2),
+          "wake": PM.wake_boxes(), "concentric": PM.concentric_boxes()}[boxes]
+    m = PM.make_nested_mesh(h, bx)
+    geo = Geometry(m)
+    t = build_tables(geo)
+    a = build_layout(m, geo, t, **kw)
+    b = build_layout_numpy(m, geo, t, **kw)
+    for name in ("cell_map", "edge_map", "edge_sigma", "edge_inv", "sigma_int", "reg_cell", "reg_edge", "reg_g1"):
+        x, y = getattr(a, name), getattr(b, name)
+        assert x.dtype == y.dtype and np.array_equal(x, y), name
+    for comp in ("g1", "vis", "g3"):
+        x, y = getattr(a, comp), getattr(b, comp)
+        for f in ("rows", "ptr", "col", "coef"):
+            u, v = getattr(x, f), getattr(y, f)
+            assert u.dtype == v.dtype and u.shape == v.shape and np.array_equal(u, v), (comp, f)
+    assert a.stats == b.stats
+    assert (a.ncell, a.nedge) == (b.ncell, b.nedge)
+    for la, lb in zip(a.levels, b.levels):
+        assert (la.nx, la.ny, la.nz, la.h, la.lo, la.hole_lo, la.hole_hi, la.cell_off, la.e_off) == \
+               (lb.nx, lb.ny, lb.nz, lb.h, lb.lo, lb.hole_lo, lb.hole_hi, lb.cell_off, lb.e_off)
+
+
+def test_numpy_reduceat_order_is_first_plus_pairwise():
+    """The summation order the C++ builder restates: add.reduceat over a run gives the first
+    entry plus NumPy's pairwise sum of the rest (checked on runs chosen so that a sequential or
+    plain pairwise order gives a different answer)."""
+    a = np.array([1.0, 1e16, -1e16])
+    assert np.add.reduceat(a, [0])[0] == 1.0 and np.add.reduce(a) == 0.0
+    rng = np.random.default_rng(3)
+    for n in (2, 7, 8, 9, 16, 17, 127, 129, 300):
+        v = rng.standard_normal(n) * 10.0 ** rng.integers(-8, 8, n)
+        rest = v[1:]
+        if rest.shape[0] < 8:
+            p = 0.0
+            for x in rest:
+                p += x
+        else:
+            p = np.add.reduce(rest)          # NumPy's own pairwise sum of a contiguous run
+        assert np.add.reduceat(v, [0])[0] == v[0] + p
