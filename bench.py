@@ -354,6 +354,86 @@ def kernels_only(n_iter, precision, log):
     return 0
 
 
+def _kernel_label(name):
+    """`pyib::k_name<T>` (or the name itself) from a demangled kernel signature."""
+    import re
+    m = re.search(r"(k_[A-Za-z0-9_]+(?:<[^()]*?>)?)\(", name)
+    return f"pyib::{m.group(1)}" if m and "pyib::" in name else name.split("(")[0].replace("void ", "")
+
+
+def _rocpd_rows(path, sql):
+    import sqlite3
+    con = sqlite3.connect(f"file:{path}?mode=ro", uri=True)
+    try:
+        return con.execute(sql).fetchall()
+    finally:
+        con.close()
+
+
+def summarize_profile(trace_db, fetch_db, write_db, n_iter, precision, nedge, command):
+    """Markdown summary of one PCG iteration from the rocprofv3 databases of ``--kernels-only``
+    runs: kernel averages (kernel trace), HBM bytes per dispatch (FETCH_SIZE / WRITE_SIZE, in
+    KiB, scaled by what they read for the plain copy kernel of 2^26 doubles that the run ends
+    with), and the algorithmic fraction of the 8 TB/s roofline."""
+    stats = {}
+    for name, calls, total in _rocpd_rows(trace_db, "select name, count(*), sum(duration) from kernels group by name"):
+        lab = _kernel_label(name)
+        c, t = stats.get(lab, (0, 0.0))
+        stats[lab] = (c + int(calls), t + float(total) / 1e3)           # ns -> us
+
+    def counter(db):
+        out = {}
+        for name, val in _rocpd_rows(db, "select kernel_name, value from counters_collection"):
+            lab = _kernel_label(name)
+            s, k = out.get(lab, (0.0, 0))
+            out[lab] = (s + float(val), k + 1)
+        return {k: s / n for k, (s, n) in out.items()}                    # KiB per dispatch
+
+    fetch, write = counter(fetch_db), counter(write_db)
+    copy = "pyib::k_convert<double, double>"
+    fs = 2.0 ** 19 / fetch[copy] if fetch.get(copy) else 1.0               # 512 MiB read
+    ws = 2.0 ** 19 / write[copy] if write.get(copy) else 1.0               # 512 MiB written
+    iter_k = sorted(((lab, c, t) for lab, (c, t) in stats.items() if c >= n_iter and lab.startswith("pyib::")),
+                    key=lambda x: -x[2])
+    per_iter = sum(t for _, _, t in iter_k) / n_iter
+    bpe = BYTES_PER_EDGE_ITER["float32" if precision == "mixed" else "float64"]
+    rows, rd_tot, wr_tot = [], 0.0, 0.0
+    for lab, c, t in iter_k:
+        avg = t / c
+        share = (t / n_iter) / per_iter
+        rd = fetch.get(lab, 0.0) * 1024 * fs * (c / n_iter)
+        wr = write.get(lab, 0.0) * 1024 * ws * (c / n_iter)
+        rd_tot, wr_tot = rd_tot + rd, wr_tot + wr
+        gbs = (rd + wr) / ((t / n_iter) * 1e-6) / 1e9 if t else 0.0
+        rows.append(f"| `{lab}` | {c} | {avg:.1f} | {100 * share:.1f}% | {rd / 1e6:.0f} | {wr / 1e6:.0f} | "
+                    f"{gbs:.0f} | {gbs / HBM_PEAK_GBS:.2f} |")
+    alg = bpe * nedge
+    prec_word = "float32" if precision == "mixed" else "float64"
+    lines = [
+        f"# One {prec_word} PCG iteration of the D32 headline operator (kernel trace and HBM traffic)", "",
+        "Workload: the D32 oscillating-sphere case, 3 nested levels, finest 192x160x160: "
+        f"{nedge:,} edge unknowns in the structured layout. `bench.py --kernels-only N` runs N PCG iterations of "
+        "its Helmholtz operator on a right-hand side in the operator's range, then one plain copy kernel of "
+        "2^26 doubles, which calibrates the HBM counters.", "",
+        "Commands (one MI355X, after `cd /tmp && export TMPDIR=/tmp PYIB_HIP_SETUP_CACHE=/tmp/pyib_cache`; "
+        "`$R` is the repository root):", "", *[f"    {c}" for c in command], "",
+        f"Counter calibration: the copy kernel reads and writes 512 MiB each; FETCH_SIZE is scaled by {fs:.3f} and "
+        f"WRITE_SIZE by {ws:.3f} below.", "",
+        "| kernel | calls | avg us | share of iteration | measured HBM read MB | write MB | measured GB/s | of 8 TB/s |",
+        "|---|---:|---:|---:|---:|---:|---:|---:|", *rows, "",
+        f"Per iteration: {per_iter:.1f} us (sum of the kernel times over {n_iter} iterations); measured HBM traffic "
+        f"{(rd_tot + wr_tot) / 1e9:.3f} GB ({(rd_tot + wr_tot) / alg:.2f}x the algorithmic bytes), i.e. "
+        f"{(rd_tot + wr_tot) / (per_iter * 1e-6) / 1e12:.2f} TB/s; algorithmic bytes {alg / 1e9:.3f} GB "
+        f"({bpe:.0f} B per edge unknown), i.e. {alg / (per_iter * 1e-6) / 1e12:.2f} TB/s = "
+        f"{alg / (per_iter * 1e-6) / 1e9 / HBM_PEAK_GBS:.3f} of the 8 TB/s roofline.", "",
+        f"traffic_bytes_per_iteration {prec_word} = {rd_tot + wr_tot:.6e}", "",
+        "Other kernels in the same trace (set-up of the right-hand side and the copy kernel):", "",
+        *[f"- `{lab}`: {c} call(s), {t / c:.1f} us avg" for lab, (c, t) in
+          sorted(stats.items(), key=lambda kv: -kv[1][1]) if (lab, c, t) not in iter_k],
+    ]
+    return "\n".join(lines) + "\n"
+
+
 def roofline(r, prec):
     cg = r["cg"].get(prec) or next(iter(r["cg"].values()))
     bytes_iter = BYTES_PER_EDGE_ITER[prec] * r["nedge"]
@@ -380,6 +460,11 @@ def main(argv=None):
     p.add_argument("--no-reference-case", action="store_true",
                    help="skip the reference's own D32 case (steps 1-3 against its CUDA timings)")
     p.add_argument("--kernels-only", type=int, default=0, help="profiling: N PCG iterations + a copy kernel")
+    p.add_argument("--summarize-profile", nargs=3, metavar=("TRACE_DB", "FETCH_DB", "WRITE_DB"),
+                   help="print the markdown summary of --kernels-only rocprofv3 runs (kernel trace, FETCH_SIZE, "
+                        "WRITE_SIZE databases); --kernels-only gives their iteration count, --precision theirs")
+    p.add_argument("--profile-command", action="append", default=[],
+                   help="a command line to quote in the summary (repeatable)")
     p.add_argument("--mixed-inner-tol", type=float, default=None,
                    help="float32 residual reduction between float64 reliable updates (default: the schema's)")
     a = p.parse_args(argv)
@@ -392,6 +477,10 @@ def main(argv=None):
               file=sys.stderr)
         return 2
     log = lambda msg: print(msg, file=sys.stderr, flush=True)
+    if a.summarize_profile:
+        n_it = a.kernels_only or 100
+        print(summarize_profile(*a.summarize_profile, n_it, a.precision, 40986624, a.profile_command), end="")
+        return 0
     if a.kernels_only:
         return kernels_only(a.kernels_only, "mixed" if a.precision == "mixed" else "float64", log)
     precs = ["float64", "mixed"] if a.precision == "both" else [a.precision]

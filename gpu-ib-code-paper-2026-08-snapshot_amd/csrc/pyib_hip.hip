@@ -449,39 +449,76 @@ __global__ void k_ib_interp(int64_t nmark, const int32_t* __restrict__ cell, con
   }
 }
 
+constexpr int32_t kNoCell = 0x7fffffff;   // sort key of a tap outside the mesh (sorts last)
+
 __global__ void k_ib_keys(int64_t n, const int32_t* __restrict__ cell, int32_t* __restrict__ keys,
                           int32_t* __restrict__ vals) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int c = cell[i];
-    keys[i] = c < 0 ? 0x7fffffff : c;
+    keys[i] = c < 0 ? kNoCell : c;
     vals[i] = (int32_t)i;
   }
 }
 
-// Segmented spreading over cell-sorted taps: each segment head sums its run in
-// marker order (= NumPy bincount order), then writes or accumulates.   (ib_immersed.py:103-111)
+// Segmented spreading over cell-sorted taps, reduced in LDS (ib_immersed.py:103-111).
+// A block takes kBlock consecutive sorted taps: every lane gathers its tap's
+// contribution flag[m] * (w * ds3[m]) into LDS, then each segment head (first tap
+// of a cell's run) sums its run from LDS in marker order, the order NumPy's
+// bincount adds in, and writes or accumulates the cell once.  A run that goes on
+// past the block's taps is finished from global memory by the same head, so no
+// cell is written by two blocks and no atomics are used: repeated runs are
+// bitwise identical.
+
+__device__ __forceinline__ void tap_contribution(int32_t v, const double* __restrict__ w,
+                                                 const double* __restrict__ flag, const double* __restrict__ ds3,
+                                                 double& c0, double& c1, double& c2) {
+  const int m = v / 27;
+  const double wt = w[v] * ds3[m];
+  c0 = flag[3 * m] * wt; c1 = flag[3 * m + 1] * wt; c2 = flag[3 * m + 2] * wt;
+}
+
 template <typename T>
-__global__ void k_ib_spread(int64_t n, const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
-                            const double* __restrict__ w, const double* __restrict__ flag,
-                            const double* __restrict__ ds3, double scale, int accumulate, T* __restrict__ out) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int c = keys[i];
-    if (c == 0x7fffffff) continue;
-    if (i > 0 && keys[i - 1] == c) continue;
-    double s0 = 0, s1 = 0, s2 = 0;
-    for (int64_t j = i; j < n && keys[j] == c; ++j) {
-      const int v = vals[j];
-      const int m = v / 27;
-      const double wt = w[v] * ds3[m];
-      s0 += flag[3 * m] * wt; s1 += flag[3 * m + 1] * wt; s2 += flag[3 * m + 2] * wt;
+__global__ void __launch_bounds__(kBlock) k_ib_spread(int64_t n, const int32_t* __restrict__ keys,
+                                                      const int32_t* __restrict__ vals, const double* __restrict__ w,
+                                                      const double* __restrict__ flag, const double* __restrict__ ds3,
+                                                      double scale, int accumulate, T* __restrict__ out) {
+  __shared__ int32_t skey[kBlock];
+  __shared__ double sc[3][kBlock];
+  const int64_t nchunk = (n + kBlock - 1) / kBlock;
+  for (int64_t chunk = blockIdx.x; chunk < nchunk; chunk += gridDim.x) {   // block-uniform trip count
+    const int64_t b0 = chunk * kBlock;
+    const int t = threadIdx.x;
+    const int64_t i = b0 + t;
+    int32_t c = kNoCell;
+    double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+    if (i < n) {
+      c = keys[i];
+      if (c != kNoCell) tap_contribution(vals[i], w, flag, ds3, c0, c1, c2);
     }
-    if (accumulate) {
-      out[3 * c] = (T)((double)out[3 * c] + scale * s0);
-      out[3 * c + 1] = (T)((double)out[3 * c + 1] + scale * s1);
-      out[3 * c + 2] = (T)((double)out[3 * c + 2] + scale * s2);
-    } else {
-      out[3 * c] = (T)(scale * s0); out[3 * c + 1] = (T)(scale * s1); out[3 * c + 2] = (T)(scale * s2);
+    skey[t] = c;
+    sc[0][t] = c0; sc[1][t] = c1; sc[2][t] = c2;
+    __syncthreads();
+    const bool head = c != kNoCell && (t > 0 ? skey[t - 1] != c : (i == 0 || keys[i - 1] != c));
+    if (head) {
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+      int j = t;
+      for (; j < kBlock && skey[j] == c; ++j) { s0 += sc[0][j]; s1 += sc[1][j]; s2 += sc[2][j]; }
+      if (j == kBlock) {                      // the run continues into the next block's taps
+        for (int64_t g = b0 + kBlock; g < n && keys[g] == c; ++g) {
+          double d0, d1, d2;
+          tap_contribution(vals[g], w, flag, ds3, d0, d1, d2);
+          s0 += d0; s1 += d1; s2 += d2;
+        }
+      }
+      if (accumulate) {
+        out[3 * c] = (T)((double)out[3 * c] + scale * s0);
+        out[3 * c + 1] = (T)((double)out[3 * c + 1] + scale * s1);
+        out[3 * c + 2] = (T)((double)out[3 * c + 2] + scale * s2);
+      } else {
+        out[3 * c] = (T)(scale * s0); out[3 * c + 1] = (T)(scale * s1); out[3 * c + 2] = (T)(scale * s2);
+      }
     }
+    __syncthreads();                          // LDS is rewritten by the next chunk
   }
 }
 

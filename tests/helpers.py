@@ -68,8 +68,8 @@ def parity_report(tag, err, floor, base=1e-12, factor=3.0):
     """Print a parity measurement with its ratio to the oracle's own two-path floor and
     return whether it is within parity_tol."""
     ratio = err / floor if floor > 0 else float("inf")
-    print(f"[gpu] {tag}: {err:.3e} (floor {floor:.3e}, ratio {ratio:.2f}, "
-          f"allowance {parity_tol(floor, base, factor):.3e})", flush=True)
+    progress(f"[gpu] {tag}: {err:.3e} (floor {floor:.3e}, ratio {ratio:.2f}, "
+             f"allowance {parity_tol(floor, base, factor):.3e})")
     return err <= parity_tol(floor, base, factor)
 
 
