@@ -67,43 +67,54 @@ inline bool uses_prec_classes(const pyib_grid* g, const void* prec) {
 // memory round trips per batch instead of two per entry.  Entries are summed in order.
 constexpr int kSellBatch = 8;
 
-__device__ __forceinline__ int64_t wave_slice() {
-  return __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+// first slice of this wave: wave w of composite block blk (wave-uniform, in a scalar register)
+__device__ __forceinline__ int64_t block_wave(int64_t blk) {
+  return __builtin_amdgcn_readfirstlane((int)(blk * (blockDim.x >> 6) + (threadIdx.x >> 6)));
 }
 
 // ---------------------------------------------------------------- G1
 // vc_d(row) = sum_j coef_d * p[col]
+// The row bodies take the block's index among the composite blocks (blk of nblk), so a
+// launch can also run them beside the tiled pass (see the merged kernels below); B is the
+// gather batch (entries are summed in order, so B changes no result, only registers).
+template <typename T, int B>
+__device__ __forceinline__ void sell_g1_rows(int64_t blk, int64_t nblk, int64_t nslice, const int32_t* __restrict__ row,
+                                             const int32_t* __restrict__ width, const int64_t* __restrict__ sptr,
+                                             const int32_t* __restrict__ col, const T* __restrict__ coef,
+                                             const T* __restrict__ p, T* __restrict__ vc) {
+  const int lane = threadIdx.x & 63;
+  for (int64_t s = block_wave(blk); s < nslice; s += nblk * (blockDim.x >> 6)) {
+    const int W = width[s];
+    const int64_t b = sptr[s] + lane;
+    const int r = row[s * 64 + lane];
+    T a0 = 0, a1 = 0, a2 = 0;
+    for (int j0 = 0; j0 < W; j0 += B) {
+      int cc[B];
+      T c0[B], c1[B], c2[B], xv[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        if (j0 + u < W) {
+          const int64_t e = b + (int64_t)(j0 + u) * 64;
+          cc[u] = col[e];
+          c0[u] = coef[3 * e]; c1[u] = coef[3 * e + 1]; c2[u] = coef[3 * e + 2];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+        if (j0 + u < W) xv[u] = p[cc[u]];
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+        if (j0 + u < W) { a0 += c0[u] * xv[u]; a1 += c1[u] * xv[u]; a2 += c2[u] * xv[u]; }
+    }
+    if (r >= 0) { vc[3 * (int64_t)r] = a0; vc[3 * (int64_t)r + 1] = a1; vc[3 * (int64_t)r + 2] = a2; }
+  }
+}
 template <typename T>
 __global__ void k_sell_g1(int64_t nslice, int64_t ncell, const int32_t* __restrict__ row,
                           const int32_t* __restrict__ width, const int64_t* __restrict__ sptr,
                           const int32_t* __restrict__ col, const T* __restrict__ coef, const T* __restrict__ p,
                           T* __restrict__ vc) {
-  const int lane = threadIdx.x & 63;
-  const int64_t s = wave_slice();
-  if (s >= nslice) return;
-  const int W = width[s];
-  const int64_t b = sptr[s] + lane;
-  const int r = row[s * 64 + lane];
-  T a0 = 0, a1 = 0, a2 = 0;
-  for (int j0 = 0; j0 < W; j0 += kSellBatch) {
-    int cc[kSellBatch];
-    T c0[kSellBatch], c1[kSellBatch], c2[kSellBatch], xv[kSellBatch];
-#pragma unroll
-    for (int u = 0; u < kSellBatch; ++u) {
-      if (j0 + u < W) {
-        const int64_t e = b + (int64_t)(j0 + u) * 64;
-        cc[u] = col[e];
-        c0[u] = coef[3 * e]; c1[u] = coef[3 * e + 1]; c2[u] = coef[3 * e + 2];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kSellBatch; ++u)
-      if (j0 + u < W) xv[u] = p[cc[u]];
-#pragma unroll
-    for (int u = 0; u < kSellBatch; ++u)
-      if (j0 + u < W) { a0 += c0[u] * xv[u]; a1 += c1[u] * xv[u]; a2 += c2[u] * xv[u]; }
-  }
-  if (r >= 0) { vc[3 * (int64_t)r] = a0; vc[3 * (int64_t)r + 1] = a1; vc[3 * (int64_t)r + 2] = a2; }
+  sell_g1_rows<T, kSellBatch>(blockIdx.x, gridDim.x, nslice, row, width, sptr, col, coef, p, vc);
 }
 
 // ----------------------------------------------------------------- V
@@ -111,25 +122,25 @@ __global__ void k_sell_g1(int64_t nslice, int64_t ncell, const int32_t* __restri
 // sum_row rw(row) x(row).y(row) (rw = the row's cell volume): these rows' share of p.A p,
 // since A = rot integr V interp curl with integr = interp^T diag(1/vol_ci) makes
 // p.A p = sum_cells vol (interp curl p).(V interp curl p).
-template <typename T>
-__global__ void k_sell_v(int64_t nslice, int64_t ncell, const int32_t* __restrict__ row,
-                         const int32_t* __restrict__ width, const int64_t* __restrict__ sptr,
-                         const int32_t* __restrict__ col, const T* __restrict__ coef, T rou, T kv,
-                         const T* __restrict__ x, T* __restrict__ y, const double* __restrict__ rw,
-                         double* __restrict__ partials) {
+template <typename T, int B>
+__device__ __forceinline__ void sell_v_rows(int64_t blk, int64_t nblk, int64_t nslice, const int32_t* __restrict__ row,
+                                            const int32_t* __restrict__ width, const int64_t* __restrict__ sptr,
+                                            const int32_t* __restrict__ col, const T* __restrict__ coef, T rou, T kv,
+                                            const T* __restrict__ x, T* __restrict__ y, const double* __restrict__ rw,
+                                            double* __restrict__ partials) {
   __shared__ double sh[kFinBlock / 64];
   const int lane = threadIdx.x & 63;
   double dot = 0.0;
-  for (int64_t s = wave_slice(); s < nslice; s += (int64_t)gridDim.x * (blockDim.x >> 6)) {
+  for (int64_t s = block_wave(blk); s < nslice; s += nblk * (blockDim.x >> 6)) {
     const int W = width[s];
     const int64_t b = sptr[s] + lane;
     const int r = row[s * 64 + lane];
     T a0 = 0, a1 = 0, a2 = 0;
-    for (int j0 = 0; j0 < W; j0 += kSellBatch) {
-      int cc[kSellBatch];
-      T cf[kSellBatch], x0[kSellBatch], x1[kSellBatch], x2[kSellBatch];
+    for (int j0 = 0; j0 < W; j0 += B) {
+      int cc[B];
+      T cf[B], x0[B], x1[B], x2[B];
 #pragma unroll
-      for (int u = 0; u < kSellBatch; ++u) {
+      for (int u = 0; u < B; ++u) {
         if (j0 + u < W) {
           const int64_t e = b + (int64_t)(j0 + u) * 64;
           cc[u] = col[e];
@@ -137,13 +148,13 @@ __global__ void k_sell_v(int64_t nslice, int64_t ncell, const int32_t* __restric
         }
       }
 #pragma unroll
-      for (int u = 0; u < kSellBatch; ++u)
+      for (int u = 0; u < B; ++u)
         if (j0 + u < W) {
           const int64_t o = 3 * (int64_t)cc[u];
           x0[u] = x[o]; x1[u] = x[o + 1]; x2[u] = x[o + 2];
         }
 #pragma unroll
-      for (int u = 0; u < kSellBatch; ++u)
+      for (int u = 0; u < B; ++u)
         if (j0 + u < W) { a0 += cf[u] * x0[u]; a1 += cf[u] * x1[u]; a2 += cf[u] * x2[u]; }
     }
     if (r >= 0) {
@@ -159,34 +170,42 @@ __global__ void k_sell_v(int64_t nslice, int64_t ncell, const int32_t* __restric
   }
   if (partials) {
     const double rr = block_sum(dot, sh);
-    if (threadIdx.x == 0) partials[blockIdx.x] = rr;
+    if (threadIdx.x == 0) partials[blk] = rr;
   }
+}
+template <typename T>
+__global__ void k_sell_v(int64_t nslice, int64_t ncell, const int32_t* __restrict__ row,
+                         const int32_t* __restrict__ width, const int64_t* __restrict__ sptr,
+                         const int32_t* __restrict__ col, const T* __restrict__ coef, T rou, T kv,
+                         const T* __restrict__ x, T* __restrict__ y, const double* __restrict__ rw,
+                         double* __restrict__ partials) {
+  sell_v_rows<T, kSellBatch>(blockIdx.x, gridDim.x, nslice, row, width, sptr, col, coef, rou, kv, x, y, rw, partials);
 }
 
 // ---------------------------------------------------------------- G3
 // w(row) = sum_j coef . y[col] (Dirichlet rows 0).  With r non-NULL: r(row) -= alpha w(row)
-// instead of storing w, and block partials of r.prec.r over these rows (as k_sg3_t)
-template <typename T>
-__global__ void k_sell_g3(int64_t nslice, int64_t ncell, const int32_t* __restrict__ row,
-                          const int32_t* __restrict__ width, const int64_t* __restrict__ sptr,
-                          const int32_t* __restrict__ col, const T* __restrict__ coef,
-                          const uint8_t* __restrict__ mask, const T* __restrict__ y, T* __restrict__ w,
-                          const double* __restrict__ scal, T* __restrict__ rv, const T* __restrict__ prec,
-                          double* __restrict__ partials) {
+// instead of storing w, and block partials of r.prec.r over these rows (as the tiled G3 pass)
+template <typename T, int B>
+__device__ __forceinline__ void sell_g3_rows(int64_t blk, int64_t nblk, int64_t nslice, const int32_t* __restrict__ row,
+                                             const int32_t* __restrict__ width, const int64_t* __restrict__ sptr,
+                                             const int32_t* __restrict__ col, const T* __restrict__ coef,
+                                             const uint8_t* __restrict__ mask, const T* __restrict__ y,
+                                             T* __restrict__ w, const double* __restrict__ scal, T* __restrict__ rv,
+                                             const T* __restrict__ prec, double* __restrict__ partials) {
   __shared__ double sh[kFinBlock / 64];
   const int lane = threadIdx.x & 63;
   const T alf = rv ? (T)scal[S_ALPHA] : T(0);
   double dot = 0.0;
-  for (int64_t s = wave_slice(); s < nslice; s += (int64_t)gridDim.x * (blockDim.x >> 6)) {
+  for (int64_t s = block_wave(blk); s < nslice; s += nblk * (blockDim.x >> 6)) {
     const int W = width[s];
     const int64_t b = sptr[s] + lane;
     const int r = row[s * 64 + lane];
     T acc = 0;
-    for (int j0 = 0; j0 < W; j0 += kSellBatch) {
-      int cc[kSellBatch];
-      T c0[kSellBatch], c1[kSellBatch], c2[kSellBatch], y0[kSellBatch], y1[kSellBatch], y2[kSellBatch];
+    for (int j0 = 0; j0 < W; j0 += B) {
+      int cc[B];
+      T c0[B], c1[B], c2[B], y0[B], y1[B], y2[B];
 #pragma unroll
-      for (int u = 0; u < kSellBatch; ++u) {
+      for (int u = 0; u < B; ++u) {
         if (j0 + u < W) {
           const int64_t e = b + (int64_t)(j0 + u) * 64;
           cc[u] = col[e];
@@ -194,13 +213,13 @@ __global__ void k_sell_g3(int64_t nslice, int64_t ncell, const int32_t* __restri
         }
       }
 #pragma unroll
-      for (int u = 0; u < kSellBatch; ++u)
+      for (int u = 0; u < B; ++u)
         if (j0 + u < W) {
           const int64_t o = 3 * (int64_t)cc[u];
           y0[u] = y[o]; y1[u] = y[o + 1]; y2[u] = y[o + 2];
         }
 #pragma unroll
-      for (int u = 0; u < kSellBatch; ++u) {
+      for (int u = 0; u < B; ++u) {
         if (j0 + u < W) {
           T t = c0[u] * y0[u] + c1[u] * y1[u];
           t = t + c2[u] * y2[u];
@@ -221,8 +240,18 @@ __global__ void k_sell_g3(int64_t nslice, int64_t ncell, const int32_t* __restri
   }
   if (partials) {
     const double tot = block_sum(dot, sh);
-    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+    if (threadIdx.x == 0) partials[blk] = tot;
   }
+}
+template <typename T>
+__global__ void k_sell_g3(int64_t nslice, int64_t ncell, const int32_t* __restrict__ row,
+                          const int32_t* __restrict__ width, const int64_t* __restrict__ sptr,
+                          const int32_t* __restrict__ col, const T* __restrict__ coef,
+                          const uint8_t* __restrict__ mask, const T* __restrict__ y, T* __restrict__ w,
+                          const double* __restrict__ scal, T* __restrict__ rv, const T* __restrict__ prec,
+                          double* __restrict__ partials) {
+  sell_g3_rows<T, kSellBatch>(blockIdx.x, gridDim.x, nslice, row, width, sptr, col, coef, mask, y, w, scal, rv, prec,
+                              partials);
 }
 
 // ------------------------------------------------- 2.5D tiled stencil passes
@@ -232,9 +261,17 @@ __global__ void k_sell_g3(int64_t nslice, int64_t ncell, const int32_t* __restri
 // operand is read from memory about once instead of once per stencil tap.  Values outside
 // the level box are staged as 0 (only rows away from the box edge are regular).  Only the
 // regular rows are written; the composite pass writes the others.
-constexpr int TJ = 8, TK = 32, CI = 16;        // 256 threads: tj = tid / TK, tk = tid % TK
+// i-planes per block of the G1/V passes (CI) and of the G3 pass (CI3): measured at D32, 16 planes
+// suit the G1/V passes and 8 the G3 pass (240 -> 226 us in float64; 32 planes are slower in both)
+#ifndef PYIB_TILE_CI
+#define PYIB_TILE_CI 16
+#endif
+#ifndef PYIB_TILE_CI3
+#define PYIB_TILE_CI3 8
+#endif
+constexpr int TJ = 8, TK = 32, CI = PYIB_TILE_CI, CI3 = PYIB_TILE_CI3;   // 256 threads: tj = tid / TK, tk = tid % TK
 
-// grid of the tiled passes; *ci = planes per block (CI, doubled until the block count fits
+// grid of the tiled passes; *ci = planes per block (given, doubled until the block count fits
 // the reduction partials)
 inline dim3 tile_grid(const Levels& L, int* ci) {
   int my = 1, mz = 1, mx = 1;
@@ -244,7 +281,7 @@ inline dim3 tile_grid(const Levels& L, int* ci) {
     mx = L.l[i].nx > mx ? L.l[i].nx : mx;
   }
   const unsigned gx = (unsigned)(((my + TJ - 1) / TJ) * ((mz + TK - 1) / TK));
-  int c = CI;
+  int c = *ci;
   while ((int64_t)gx * ((mx + c - 1) / c) * L.n > kRedGrid) c *= 2;
   *ci = c;
   return dim3(gx, (unsigned)((mx + c - 1) / c), (unsigned)L.n);
@@ -257,14 +294,18 @@ struct Tile {
 // tile of this block for a level whose (j, k) extent is (ny_t, nz_t) and i extent ni
 // block ids remapped so that each XCD (workgroups go round-robin over the 8 XCDs) gets one
 // contiguous run of tiles: neighbouring tiles share that XCD's L2 for their halo lines
-__device__ __forceinline__ void xcd_ids(int& bx, int& by, int& bz) {
-  const int64_t gx = gridDim.x, gy = gridDim.y, nb = gx * gy * gridDim.z;
-  const int64_t b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+// b = the block's linear dispatch index among the tile grid tg's blocks
+__device__ __forceinline__ void xcd_ids_of(int64_t b, const dim3& tg, int& bx, int& by, int& bz) {
+  const int64_t gx = tg.x, gy = tg.y, nb = gx * gy * tg.z;
   const int64_t q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
   const int64_t lb = x * q + (x < r ? x : r) + i;
   bx = (int)(lb % gx);
   by = (int)((lb / gx) % gy);
   bz = (int)(lb / (gx * gy));
+}
+__device__ __forceinline__ void xcd_ids(int& bx, int& by, int& bz) {
+  xcd_ids_of(blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z),
+             dim3(gridDim.x, gridDim.y, gridDim.z), bx, by, bz);
 }
 
 // every cell the tile owns lies in the level's hole: no regular row (G1, V or G3) belongs to
@@ -300,10 +341,10 @@ struct Prefetch {
 // ---- G1: vc = interp(curl p) on the cells of plane i (reads x-edges of plane i, y/z-edges
 // of planes i and i+1)
 template <typename T>
-__global__ __launch_bounds__(256) void k_sg1_t(int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
+__device__ __forceinline__ void sg1_tile(int64_t lb, const dim3& tg, int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
                                                const T* __restrict__ p, T* __restrict__ vc) {
   int bx, by, bz;
-  xcd_ids(bx, by, bz);
+  xcd_ids_of(lb, tg, bx, by, bz);
   const Lev& v = L.l[bz];
   const Tile t = tile_of(v, ci, bx, by);
   if (!t.live) return;
@@ -617,12 +658,12 @@ __global__ __launch_bounds__(256) void k_sg1v_t(int64_t ncell, Levels L, int ci,
 // With partials, block partials of h^3 sum x.y over the regular cells (their share of p.A p,
 // as in k_sell_v; vol = h^3 on a regular cell)
 template <typename T>
-__global__ __launch_bounds__(256) void k_sv_t(int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
+__device__ __forceinline__ void sv_tile(int64_t lb, const dim3& tg, int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
                                               T rou, T visc, T hdt, const T* __restrict__ x, T* __restrict__ y,
                                               double* __restrict__ partials) {
   __shared__ double sh[kFinBlock / 64];
   int bx, by, bz;
-  xcd_ids(bx, by, bz);
+  xcd_ids_of(lb, tg, bx, by, bz);
   const Lev& v = L.l[bz];
   const Tile t = tile_of(v, ci, bx, by);
   double dot = 0.0;
@@ -707,7 +748,7 @@ __global__ __launch_bounds__(256) void k_sv_t(int64_t ncell, Levels L, int ci, c
   if (partials) {
     const double r = block_sum(dot, sh);
     if (threadIdx.x == 0)
-      partials[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = r * (v.h * v.h * v.h);
+      partials[lb] = r * (v.h * v.h * v.h);
   }
 }
 
@@ -718,13 +759,13 @@ __global__ __launch_bounds__(256) void k_sv_t(int64_t ncell, Levels L, int ci, c
 // same rows (alpha = scal[S_ALPHA], already final since p.A p comes from the V pass) and
 // block partials of r.prec.r go to partials[block]
 template <typename T>
-__global__ __launch_bounds__(256) void k_sg3_t(int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
+__device__ __forceinline__ void sg3_tile(int64_t lb, const dim3& tg, int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
                                                const T* __restrict__ y, T* __restrict__ w,
                                                const double* __restrict__ scal, T* __restrict__ rr,
                                                const T* __restrict__ prec, double* __restrict__ partials) {
   __shared__ double sh[kFinBlock / 64];
   int bx, by, bz;
-  xcd_ids(bx, by, bz);
+  xcd_ids_of(lb, tg, bx, by, bz);
   const Lev& v = L.l[bz];
   const Tile t = tile_of(v, ci, bx, by);
   double dot = 0.0;
@@ -852,8 +893,62 @@ __global__ __launch_bounds__(256) void k_sg3_t(int64_t ncell, Levels L, int ci, 
   }
   if (partials) {
     const double s = block_sum(dot, sh);
-    if (threadIdx.x == 0) partials[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+    if (threadIdx.x == 0) partials[lb] = s;
   }
+}
+
+// ---- one launch per pass: the tiled stencil blocks [0, ntile) and, after them, the composite-row
+// blocks [ntile, ntile + ncomp).  Both write disjoint rows from the same inputs, so the composite
+// blocks fill the CUs the last tiles leave idle instead of waiting for the whole tiled pass.  The
+// tile blocks keep their dispatch index (XCD placement and partial slots are those of a launch of
+// the tile grid alone) and the composite blocks their index among themselves, so every sum is
+// formed in the same order as by two launches.  The composite rows gather in batches of
+// kMergedBatch, which keeps the merged kernel's registers near the tiled pass's own.
+constexpr int kMergedBatch = 4;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_g1_m(int64_t ntile, dim3 tg, int64_t ncell, Levels L, int ci,
+                                              const uint8_t* __restrict__ reg, int64_t nslice,
+                                              const int32_t* __restrict__ row, const int32_t* __restrict__ width,
+                                              const int64_t* __restrict__ sptr, const int32_t* __restrict__ col,
+                                              const T* __restrict__ coef, int64_t ncomp, const T* __restrict__ p,
+                                              T* __restrict__ vc) {
+  if (blockIdx.x < ntile)
+    sg1_tile<T>(blockIdx.x, tg, ncell, L, ci, reg, p, vc);
+  else
+    sell_g1_rows<T, kMergedBatch>(blockIdx.x - ntile, ncomp, nslice, row, width, sptr, col, coef, p, vc);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_v_m(int64_t ntile, dim3 tg, int64_t ncell, Levels L, int ci,
+                                             const uint8_t* __restrict__ reg, T rou, T visc, T hdt, int64_t nslice,
+                                             const int32_t* __restrict__ row, const int32_t* __restrict__ width,
+                                             const int64_t* __restrict__ sptr, const int32_t* __restrict__ col,
+                                             const T* __restrict__ coef, const double* __restrict__ rw,
+                                             int64_t ncomp, const T* __restrict__ x, T* __restrict__ y,
+                                             double* __restrict__ tpart, double* __restrict__ cpart) {
+  if (blockIdx.x < ntile)
+    sv_tile<T>(blockIdx.x, tg, ncell, L, ci, reg, rou, visc, hdt, x, y, tpart);
+  else
+    sell_v_rows<T, kMergedBatch>(blockIdx.x - ntile, ncomp, nslice, row, width, sptr, col, coef, rou, hdt * visc, x,
+                                 y, rw, cpart);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_g3_m(int64_t ntile, dim3 tg, int64_t ncell, Levels L, int ci,
+                                              const uint8_t* __restrict__ reg, int64_t nslice,
+                                              const int32_t* __restrict__ row, const int32_t* __restrict__ width,
+                                              const int64_t* __restrict__ sptr, const int32_t* __restrict__ col,
+                                              const T* __restrict__ coef, const uint8_t* __restrict__ mask,
+                                              int64_t ncomp, const T* __restrict__ y, T* __restrict__ w,
+                                              const double* __restrict__ scal, T* __restrict__ rr,
+                                              const T* __restrict__ prec, double* __restrict__ tpart,
+                                              double* __restrict__ cpart) {
+  if (blockIdx.x < ntile)
+    sg3_tile<T>(blockIdx.x, tg, ncell, L, ci, reg, y, w, scal, rr, prec, tpart);
+  else
+    sell_g3_rows<T, kMergedBatch>(blockIdx.x - ntile, ncomp, nslice, row, width, sptr, col, coef, mask, y, w, scal,
+                                  rr, prec, cpart);
 }
 
 // sum of two partial sets (nb1 from the composite rows, nb2 from the stencil rows)
@@ -877,9 +972,10 @@ __global__ void k_alpha2(const double* __restrict__ pa, int na, const double* __
 }
 
 // ------------------------------------------------------ CG on the grid
-// eta_new = r.prec.r from the two partial rows of the fused G3 passes, beta = eta_new / eta
+// eta_new = r.prec.r from the two partial rows of the fused G3 passes, beta = eta_new / eta; with
+// shift, also eta <- eta_new (the direction update that follows reads only alpha and beta)
 __global__ void k_gcg_beta(const double* __restrict__ pa, int na, const double* __restrict__ pb, int nb,
-                           double* __restrict__ scal) {
+                           double* __restrict__ scal, int shift) {
   __shared__ double sh[kFinBlock / 64];
   double acc = 0.0;
   for (int b = threadIdx.x; b < na; b += blockDim.x) acc += pa[b];
@@ -891,6 +987,7 @@ __global__ void k_gcg_beta(const double* __restrict__ pa, int na, const double* 
     scal[S_ETA_NEW] = en;
     const double eta = scal[S_ETA];
     scal[S_BETA] = en / (eta != 0.0 ? eta : __builtin_inf());
+    if (shift) scal[S_ETA] = en;
   }
 }
 template <typename T>
@@ -1076,27 +1173,35 @@ void g1_v(const pyib_grid* g, const pyib_work* ws, double rou, double visc, doub
   T* cb = static_cast<T*>(ws->cell_b);
   int ci = CI;
   const dim3 gt = tile_grid(L, &ci);
-  // the composite G1 rows first: the fused pass reads their x from HBM
-  if (g->g1_nslice > 0)
-    k_sell_g1<T><<<slice_grid(g->g1_nslice), kBlock, 0, st>>>(g->g1_nslice, g->ncell, g->g1_row, g->g1_width,
-                                                             g->g1_sptr, g->g1_col, (const T*)g->g1_coef, p, ca);
   double* tpart = want_dot ? ws->partials + kRedGrid : nullptr;
-  if constexpr (sizeof(T) == 8) {
-    // float64: one fused pass (measured 318 us against 167 + 179 us for the two passes at D32)
-    k_sg1v_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, g->cell_g1v, (T)rou, (T)visc, (T)hdt, p, ca, cb, tpart);
-  } else {
-    // float32: the two passes (110 + 99 us against 248 us fused at D32: the fused pass is
-    // latency-bound once its bytes halve)
-    k_sg1_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, g->cell_reg_g1, p, ca);
-    k_sv_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, g->cell_reg, (T)rou, (T)visc, (T)hdt, ca, cb, tpart);
-  }
+  double* cpart = want_dot ? ws->partials : nullptr;
+  const int64_t ntile = (int64_t)gt.x * gt.y * gt.z;
   const int n1 = slice_red_grid(g->v_nslice);
-  if (g->v_nslice > 0)
-    k_sell_v<T><<<n1, kBlock, 0, st>>>(g->v_nslice, g->ncell, g->v_row, g->v_width, g->v_sptr, g->v_col,
-                                       (const T*)g->v_coef, (T)rou, (T)(hdt * visc), ca, cb, g->v_rw,
-                                       want_dot ? ws->partials : nullptr);
-  *nv = g->v_nslice > 0 ? n1 : 0;
-  *nt = (int)(gt.x * gt.y * gt.z);
+  const int64_t nc1 = g->v_nslice > 0 ? n1 : 0;
+  if constexpr (sizeof(T) == 8) {
+    // float64: the composite G1 rows first (the fused pass reads their x from HBM), then one
+    // fused G1+V pass, then the composite V rows (they read the x the fused pass stores)
+    if (g->g1_nslice > 0)
+      k_sell_g1<T><<<slice_grid(g->g1_nslice), kBlock, 0, st>>>(g->g1_nslice, g->ncell, g->g1_row, g->g1_width,
+                                                               g->g1_sptr, g->g1_col, (const T*)g->g1_coef, p, ca);
+    k_sg1v_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, g->cell_g1v, (T)rou, (T)visc, (T)hdt, p, ca, cb, tpart);
+    if (nc1 > 0)
+      k_sell_v<T><<<n1, kBlock, 0, st>>>(g->v_nslice, g->ncell, g->v_row, g->v_width, g->v_sptr, g->v_col,
+                                         (const T*)g->v_coef, (T)rou, (T)(hdt * visc), ca, cb, g->v_rw, cpart);
+  } else {
+    // float32: G1 (tiles + composite rows) then V (tiles + composite rows), x through HBM; the
+    // fused pass is latency-bound once its bytes halve
+    const int64_t nc0 = g->g1_nslice > 0 ? slice_grid(g->g1_nslice) : 0;
+    k_g1_m<T><<<(unsigned)(ntile + nc0), kBlock, 0, st>>>(ntile, gt, g->ncell, L, ci, g->cell_reg_g1, g->g1_nslice,
+                                                         g->g1_row, g->g1_width, g->g1_sptr, g->g1_col,
+                                                         (const T*)g->g1_coef, nc0, p, ca);
+    k_v_m<T><<<(unsigned)(ntile + nc1), kBlock, 0, st>>>(ntile, gt, g->ncell, L, ci, g->cell_reg, (T)rou, (T)visc,
+                                                        (T)hdt, g->v_nslice, g->v_row, g->v_width, g->v_sptr,
+                                                        g->v_col, (const T*)g->v_coef, g->v_rw, nc1, ca, cb, tpart,
+                                                        cpart);
+  }
+  *nv = (int)nc1;
+  *nt = (int)ntile;
 }
 
 // G3 pass from ws->cell_b: w = rot integr y, or with rr non-NULL the fused residual update
@@ -1108,17 +1213,17 @@ void g3(const pyib_grid* g, const pyib_work* ws, T* w, const double* scal, T* rr
   const bool cls = rr && uses_prec_classes(g, prec);
   const Levels L = levels_of(g, cls);
   const T* cb = static_cast<const T*>(ws->cell_b);
-  int ci = CI;
+  int ci = CI3;
   const dim3 gt = tile_grid(L, &ci);
   const int n3 = slice_red_grid(g->g3_nslice);
-  k_sg3_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, cls ? g->edge_cls : g->edge_reg, cb, w, scal, rr, prec,
-                                    rr ? ws->partials + 2 * kRedGrid : nullptr);
-  if (g->g3_nslice > 0)
-    k_sell_g3<T><<<n3, kBlock, 0, st>>>(g->g3_nslice, g->ncell, g->g3_row, g->g3_width, g->g3_sptr, g->g3_col,
-                                        (const T*)g->g3_coef, g->dir, cb, w, scal, rr, prec,
-                                        rr ? ws->partials + 3 * kRedGrid : nullptr);
-  if (nt) *nt = (int)(gt.x * gt.y * gt.z);
-  if (ns) *ns = g->g3_nslice > 0 ? n3 : 0;
+  const int64_t ntile = (int64_t)gt.x * gt.y * gt.z, nc3 = g->g3_nslice > 0 ? n3 : 0;
+  k_g3_m<T><<<(unsigned)(ntile + nc3), kBlock, 0, st>>>(ntile, gt, g->ncell, L, ci, cls ? g->edge_cls : g->edge_reg,
+                                                       g->g3_nslice, g->g3_row, g->g3_width, g->g3_sptr, g->g3_col,
+                                                       (const T*)g->g3_coef, g->dir, nc3, cb, w, scal, rr, prec,
+                                                       rr ? ws->partials + 2 * kRedGrid : nullptr,
+                                                       rr ? ws->partials + 3 * kRedGrid : nullptr);
+  if (nt) *nt = (int)ntile;
+  if (ns) *ns = (int)nc3;
 }
 
 // w = A p on the grid (see g1_v for the dot partials)
@@ -1149,15 +1254,15 @@ int gcg_iterate(const pyib_grid* g, const pyib_work* ws, const pyib_cg* cg, doub
     g1_v<T>(g, ws, rou, visc, hdt, p, true, &nv, &nt, st);
     k_alpha2<<<1, kFinBlock, 0, st>>>(ws->partials, nv, ws->partials + kRedGrid, nt, cg->scal, 1, nullptr);
     g3<T>(g, ws, nullptr, cg->scal, r, prec, &nt3, &ns3, st);
+    const bool last = skip_last && it == n_iter - 1;
     k_gcg_beta<<<1, kFinBlock, 0, st>>>(ws->partials + 2 * kRedGrid, nt3, ws->partials + 3 * kRedGrid, ns3,
-                                        cg->scal);
-    if (!(skip_last && it == n_iter - 1)) {
+                                        cg->scal, last ? 0 : 1);
+    if (!last) {
       if (cls)
         k_gcg_dir_x_cls<T><<<grid_for((n + DirGroup<T>::W - 1) / DirGroup<T>::W), kBlock, 0, st>>>(
             n, cg->scal, g->edge_cls, pt, r, prec, p, x);
       else
         k_gcg_dir_x<T><<<grid_for(n), kBlock, 0, st>>>(n, cg->scal, r, prec, p, x);
-      k_gcg_shift<<<1, 1, 0, st>>>(cg->scal);
     } else {
       k_gcg_x<T><<<grid_for(n), kBlock, 0, st>>>(n, cg->scal, p, x);   // x current for the host's test
     }

@@ -98,5 +98,11 @@ def cached(kind, params, build, cls, registry=None):
     if (path / "meta.json").exists():
         return load(cls, path, registry or {})
     obj = build()
-    save(obj, path)
+    try:
+        save(obj, path)
+    except OSError as exc:          # a full or read-only cache disk only costs the next rebuild
+        import shutil
+        import warnings
+        shutil.rmtree(path, ignore_errors=True)
+        warnings.warn(f"setup cache: {kind} not stored in {d} ({exc})", RuntimeWarning, stacklevel=2)
     return obj

@@ -136,8 +136,9 @@ def test_cfg3_headline_d32():
 def test_cfg4_d40_mixed_vs_float64():
     """cfg4 (26.4 M cells, 79.9 M edges), 1 step: mixed precision against float64 on the body
     force (<= 1e-6 relative), the edge vorticity and the face flux (relative L2, bounded by
-    3e-5: both solvers stop at the same 1e-10/1e-9 test on the float64 true residual, so their
-    solutions differ by the tolerance times the conditioning, not by float32 round-off)."""
+    3e-5).  Both solvers stop at the same 1e-10/1e-9 test on the float64 true residual, which
+    bounds the vorticity (the operator applied to s); the face flux also carries float32
+    rounding in the operator's slowest modes, which that test does not resolve."""
     c64 = full_size_config("cfg4", steps=1)
     rt, s64, vc64, r64 = _run(c64, 1)
     assert rt.core.ncell == 26400000 and rt.core.nedge == 79881280

@@ -159,10 +159,10 @@ def _forces(cfg, steps):
 
 def test_mixed_precision_force_error():
     """Mixed precision vs float64: force history to 1e-6 relative; the face-flux field error is
-    stated and bounded by 1e-5 (both solvers stop at the same 1e-10/1e-9 mixed tolerance, so the
-    two solutions differ by the solver tolerance times the operator's conditioning, not by float32
-    round-off).  The stream function itself also carries a gauge (gradient) component that the
-    equations leave free; it is printed for reference only."""
+    stated and bounded by 1e-5 (float32 rounding in the operator's slowest modes, which the
+    residual stop test does not resolve; see test_mixed_precision_field_error_sources).  The
+    stream function itself also carries a gauge (gradient) component that the equations leave
+    free; it is printed for reference only."""
     f64, s64, r64 = _forces(small_case("oscillating", steps=4), 4)
     fmx, smx, rmx = _forces(small_case("oscillating", precision="mixed", steps=4), 4)
     err = np.abs(fmx - f64).max() / np.abs(f64).max()
@@ -175,18 +175,18 @@ def test_mixed_precision_force_error():
     assert rmx.driver.helmholtz_converged_last_step
 
 
-def test_mixed_precision_field_error_is_the_solver_tolerance():
-    """The face-flux difference of mixed against float64 is the solver's stop tolerance, not float32
-    round-off: float64 at the reference's tolerance (1e-10 / 1e-9) already differs by as much from a
-    float64 solve 100x tighter (1e-12 / 1e-11), and mixed lies no further from that tight solve.  The
-    edge vorticity rot(integr(vc)) is the operator applied to s, which the stop test bounds, so its
-    differences stay near the tolerance itself."""
+def test_mixed_precision_field_error_sources():
+    """Where mixed precision and float64 differ, against a float64 solve 100x tighter (1e-12 /
+    1e-11) than the reference's tolerance (1e-10 / 1e-9).  The edge vorticity rot(integr(vc)) is
+    the operator applied to s, which the stop test bounds: both precisions lie within a few times
+    the tolerance of the tight solve.  The face flux also carries the components the residual test
+    cannot see (the operator's slowest modes, whose residual is small for a visible error): float64
+    at the reference's tolerance stays within 1e-7 of the tight solve, while mixed carries float32
+    rounding there, stated and bounded by 1e-5."""
     import functools
-    cfg = small_case("oscillating", steps=4)
 
     def run(precision, tight=False):
-        c = small_case("oscillating", precision=precision, steps=4)
-        rt = HC.build_runtime(c)
+        rt = HC.build_runtime(small_case("oscillating", precision=precision, steps=4))
         if tight:
             rt.driver.helm_solve = functools.partial(rt.driver.helm_solve, tol1=1e-12, tol2=1e-11)
         s, s_old, vc = HC.initial_state(rt)
@@ -198,15 +198,16 @@ def test_mixed_precision_field_error_is_the_solver_tolerance():
         W = rt.core.rot(rt.core.integr_c2f(vc[:rt.core.ncell].contiguous())).cpu().numpy()
         return U, W
 
-    assert cfg["runtime"]["precision"] == "float64"
     Ut, Wt = run("float64", tight=True)
     U64, W64 = run("float64")
     Umx, Wmx = run("mixed")
-    e64, emx = rel(U64, Ut), rel(Umx, Ut)
-    print(f"[gpu] face flux vs float64 at 1e-12/1e-11: float64 at 1e-10/1e-9 {e64:.2e}, mixed {emx:.2e}, "
-          f"mixed vs float64 {rel(Umx, U64):.2e}; edge vorticity: {rel(W64, Wt):.2e}, {rel(Wmx, Wt):.2e}")
-    assert e64 > 1e-9                       # the tolerance leaves a visible field difference
-    assert emx <= 5.0 * e64                 # mixed adds nothing beyond it
+    w64, wmx = rel(W64, Wt), rel(Wmx, Wt)
+    u64, umx = rel(U64, Ut), rel(Umx, Ut)
+    print(f"[gpu] against float64 at 1e-12/1e-11: edge vorticity float64 {w64:.2e}, mixed {wmx:.2e}; "
+          f"face flux float64 {u64:.2e}, mixed {umx:.2e}")
+    assert w64 <= 1e-9 and wmx <= 1e-9
+    assert u64 <= 1e-7
+    assert umx <= 1e-5
 
 
 def test_reference_d32_run_history():

@@ -47,3 +47,20 @@ def test_stencil_refuses_a_kernel_wider_than_its_taps():
     with pytest.raises(ValueError):
         HipImmersedBoundary(mesh, device="cpu", hh=0.0)
     assert np.isclose(HipImmersedBoundary(mesh, device="cpu").hh, 0.25)
+
+
+def test_setup_cache_write_failure_keeps_the_run(tmp_path, monkeypatch):
+    """A cache that cannot be written (full disk) leaves no partial entry and returns the
+    object built, with a warning."""
+    from pyib_hip import setup_cache as SC
+    monkeypatch.setenv("PYIB_HIP_SETUP_CACHE", str(tmp_path))
+
+    def full(obj, path):
+        path.mkdir(parents=True, exist_ok=True)
+        (path / "x.npy").write_bytes(b"partial")
+        raise OSError("Not enough free space")
+    monkeypatch.setattr(SC, "save", full)
+    with pytest.warns(RuntimeWarning, match="not stored"):
+        out = SC.cached("thing", {"a": 1}, lambda: {"built": True}, dict)
+    assert out == {"built": True}
+    assert list(tmp_path.iterdir()) == []
