@@ -51,7 +51,7 @@ class Grid(C.Structure):
         ("v_rw", vp),
         ("g3_nslice", C.c_int64), ("g3_row", vp), ("g3_width", vp), ("g3_sptr", vp), ("g3_col", vp), ("g3_coef", vp),
         ("edge_cls", vp), ("cls_prec", vp), ("prec_level", C.c_double * MAX_LEVELS),
-        ("prec_level_g3", C.c_double * MAX_LEVELS),
+        ("prec_level_g3", C.c_double * MAX_LEVELS), ("g3_mask", vp), ("g3_rprec", vp),
     ]
 
 

@@ -109,12 +109,60 @@ __device__ __forceinline__ void sell_g1_rows(int64_t blk, int64_t nblk, int64_t 
     if (r >= 0) { vc[3 * (int64_t)r] = a0; vc[3 * (int64_t)r + 1] = a1; vc[3 * (int64_t)r + 2] = a2; }
   }
 }
+
+
+// The same rows with each slice spread over a block's four waves: wave w sums the entries
+// j = w, w+4, w+8, ... of its lane's row and the four partial sums are added in wave order.
+// The G1 composite holds few rows (the coarse cells around each hole) of about 20 entries, so
+// one wave per slice leaves the GPU mostly idle behind long gather chains; four waves per
+// slice give four times the waves and a quarter of the chain.
 template <typename T>
-__global__ void k_sell_g1(int64_t nslice, int64_t ncell, const int32_t* __restrict__ row,
-                          const int32_t* __restrict__ width, const int64_t* __restrict__ sptr,
-                          const int32_t* __restrict__ col, const T* __restrict__ coef, const T* __restrict__ p,
-                          T* __restrict__ vc) {
-  sell_g1_rows<T, kSellBatch>(blockIdx.x, gridDim.x, nslice, row, width, sptr, col, coef, p, vc);
+__global__ __launch_bounds__(256) void k_sell_g1_split(int64_t nslice, const int32_t* __restrict__ row,
+                                                       const int32_t* __restrict__ width,
+                                                       const int64_t* __restrict__ sptr,
+                                                       const int32_t* __restrict__ col, const T* __restrict__ coef,
+                                                       const T* __restrict__ p, T* __restrict__ vc) {
+  constexpr int B = 6, NW = 4;                  // one batch covers rows of up to 24 entries
+  __shared__ T part[NW - 1][3][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int64_t s = blockIdx.x; s < nslice; s += gridDim.x) {          // block-uniform
+    const int W = width[s];
+    const int64_t b = sptr[s] + lane;
+    T a0 = 0, a1 = 0, a2 = 0;
+    for (int j0 = wv; j0 < W; j0 += NW * B) {
+      int cc[B];
+      T c0[B], c1[B], c2[B], xv[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        if (j0 + NW * u < W) {
+          const int64_t e = b + (int64_t)(j0 + NW * u) * 64;
+          cc[u] = col[e];
+          c0[u] = coef[3 * e]; c1[u] = coef[3 * e + 1]; c2[u] = coef[3 * e + 2];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+        if (j0 + NW * u < W) xv[u] = p[cc[u]];
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+        if (j0 + NW * u < W) { a0 += c0[u] * xv[u]; a1 += c1[u] * xv[u]; a2 += c2[u] * xv[u]; }
+    }
+    if (wv > 0) {
+      part[wv - 1][0][lane] = a0;
+      part[wv - 1][1][lane] = a1;
+      part[wv - 1][2][lane] = a2;
+    }
+    __syncthreads();
+    if (wv == 0) {
+      const int r = row[s * 64 + lane];
+      if (r >= 0) {
+        vc[3 * (int64_t)r] = ((a0 + part[0][0][lane]) + part[1][0][lane]) + part[2][0][lane];
+        vc[3 * (int64_t)r + 1] = ((a1 + part[0][1][lane]) + part[1][1][lane]) + part[2][1][lane];
+        vc[3 * (int64_t)r + 2] = ((a2 + part[0][2][lane]) + part[1][2][lane]) + part[2][2][lane];
+      }
+    }
+    __syncthreads();                                                   // part is reused
+  }
 }
 
 // ----------------------------------------------------------------- V
@@ -185,13 +233,16 @@ __global__ void k_sell_v(int64_t nslice, int64_t ncell, const int32_t* __restric
 // ---------------------------------------------------------------- G3
 // w(row) = sum_j coef . y[col] (Dirichlet rows 0).  With r non-NULL: r(row) -= alpha w(row)
 // instead of storing w, and block partials of r.prec.r over these rows (as the tiled G3 pass)
+// mask: Dirichlet rows forced to 0 (NULL: the rows hold none); rprec (with rv): prec at each
+// lane's row, stored with the slices, instead of the scattered prec[row] (NULL: read prec)
 template <typename T, int B>
 __device__ __forceinline__ void sell_g3_rows(int64_t blk, int64_t nblk, int64_t nslice, const int32_t* __restrict__ row,
                                              const int32_t* __restrict__ width, const int64_t* __restrict__ sptr,
                                              const int32_t* __restrict__ col, const T* __restrict__ coef,
                                              const uint8_t* __restrict__ mask, const T* __restrict__ y,
                                              T* __restrict__ w, const double* __restrict__ scal, T* __restrict__ rv,
-                                             const T* __restrict__ prec, double* __restrict__ partials) {
+                                             const T* __restrict__ prec, const T* __restrict__ rprec,
+                                             double* __restrict__ partials) {
   __shared__ double sh[kFinBlock / 64];
   const int lane = threadIdx.x & 63;
   const T alf = rv ? (T)scal[S_ALPHA] : T(0);
@@ -232,7 +283,8 @@ __device__ __forceinline__ void sell_g3_rows(int64_t blk, int64_t nblk, int64_t 
       if (rv) {
         const T ri = rv[r] - alf * acc;
         rv[r] = ri;
-        dot += (double)(ri * prec[r]) * (double)ri;
+        const T m = rprec ? rprec[s * 64 + lane] : prec[r];
+        dot += (double)(ri * m) * (double)ri;
       } else {
         w[r] = acc;
       }
@@ -243,16 +295,7 @@ __device__ __forceinline__ void sell_g3_rows(int64_t blk, int64_t nblk, int64_t 
     if (threadIdx.x == 0) partials[blk] = tot;
   }
 }
-template <typename T>
-__global__ void k_sell_g3(int64_t nslice, int64_t ncell, const int32_t* __restrict__ row,
-                          const int32_t* __restrict__ width, const int64_t* __restrict__ sptr,
-                          const int32_t* __restrict__ col, const T* __restrict__ coef,
-                          const uint8_t* __restrict__ mask, const T* __restrict__ y, T* __restrict__ w,
-                          const double* __restrict__ scal, T* __restrict__ rv, const T* __restrict__ prec,
-                          double* __restrict__ partials) {
-  sell_g3_rows<T, kSellBatch>(blockIdx.x, gridDim.x, nslice, row, width, sptr, col, coef, mask, y, w, scal, rv, prec,
-                              partials);
-}
+
 
 // ------------------------------------------------- 2.5D tiled stencil passes
 // Each block owns a TJ x TK tile of one level's (j, k) plane and marches over a chunk of
@@ -314,6 +357,15 @@ __device__ __forceinline__ void xcd_ids(int& bx, int& by, int& bz) {
 __device__ __forceinline__ bool in_hole(const Lev& v, int j0, int k0, int i0, int i1) {
   const int j1 = min(j0 + TJ, v.ny), k1 = min(k0 + TK, v.nz);
   return i0 >= v.hl[0] && i1 <= v.hh[0] && j0 >= v.hl[1] && j1 <= v.hh[1] && k0 >= v.hl[2] && k1 <= v.hh[2];
+}
+
+// number of the faces of cell (i, j, k) on the level box's boundary.  On a box-face V row (the
+// coarsest level's cells on the domain boundary) such a face has a zero ghost and AoL = 2h, twice
+// the stencil's coefficient; the staged neighbour outside the box is 0, so the face's extra share
+// is -kA x of the cell.
+template <typename T>
+__device__ __forceinline__ T box_faces(const Lev& v, int i, int j, int k) {
+  return (T)((i == 0) + (i == v.nx - 1) + (j == 0) + (j == v.ny - 1) + (k == 0) + (k == v.nz - 1));
 }
 
 __device__ __forceinline__ Tile tile_of(const Lev& v, int ci, int bx, int by) {
@@ -625,13 +677,16 @@ __global__ __launch_bounds__(256) void k_sg1v_t(int64_t ncell, Levels L, int ci,
       const int i = ip - 1;                                  // V on plane i
       if (i >= t.i0) {
         const int s0 = slot(i);
-        if (sf[s0][qo] & 4) {
+        const uint8_t fv = sf[s0][qo];
+        if (fv & 4) {
           const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
+          const T nbox = (fv & 16) ? box_faces<T>(v, i, j, k) : T(0);
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
             const T xc = sc[s0][d][qo];
             T aa = kA * (sc[a][d][qo] - xc) + kA * (sc[s0][d][qo + RK] - xc);
             aa = aa + kA * (sc[s0][d][qo + 1] - xc);
+            if (fv & 16) aa = aa - nbox * (kA * xc);
             T bb = kA * (xc - xm[d]) + kA * (xc - sc[s0][d][qo - RK]);
             bb = bb + kA * (xc - sc[s0][d][qo - 1]);
             const T yv = rou * xc - hv * (aa - bb);
@@ -716,11 +771,13 @@ __device__ __forceinline__ void sv_tile(int64_t lb, const dim3& tg, int64_t ncel
       if (g) {
         const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
         const int sm = (i + 2) % 3, s0 = i % 3, sp = (i + 1) % 3;
+        const T nbox = (g & 2) ? box_faces<T>(v, i, j, k) : T(0);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           const T xc = sx[s0][d][q];
           T a = kA * (sx[sp][d][q] - xc) + kA * (sx[s0][d][q + PK] - xc);
           a = a + kA * (sx[s0][d][q + 1] - xc);
+          if (g & 2) a = a - nbox * (kA * xc);
           T b = kA * (xc - sx[sm][d][q]) + kA * (xc - sx[s0][d][q - PK]);
           b = b + kA * (xc - sx[s0][d][q - 1]);
           const T yv = rou * xc - hv * (a - b);
@@ -942,13 +999,13 @@ __global__ __launch_bounds__(256) void k_g3_m(int64_t ntile, dim3 tg, int64_t nc
                                               const T* __restrict__ coef, const uint8_t* __restrict__ mask,
                                               int64_t ncomp, const T* __restrict__ y, T* __restrict__ w,
                                               const double* __restrict__ scal, T* __restrict__ rr,
-                                              const T* __restrict__ prec, double* __restrict__ tpart,
-                                              double* __restrict__ cpart) {
+                                              const T* __restrict__ prec, const T* __restrict__ rprec,
+                                              double* __restrict__ tpart, double* __restrict__ cpart) {
   if (blockIdx.x < ntile)
     sg3_tile<T>(blockIdx.x, tg, ncell, L, ci, reg, y, w, scal, rr, prec, tpart);
   else
     sell_g3_rows<T, kMergedBatch>(blockIdx.x - ntile, ncomp, nslice, row, width, sptr, col, coef, mask, y, w, scal,
-                                  rr, prec, cpart);
+                                  rr, prec, rprec, cpart);
 }
 
 // sum of two partial sets (nb1 from the composite rows, nb2 from the stencil rows)
@@ -1182,8 +1239,9 @@ void g1_v(const pyib_grid* g, const pyib_work* ws, double rou, double visc, doub
     // float64: the composite G1 rows first (the fused pass reads their x from HBM), then one
     // fused G1+V pass, then the composite V rows (they read the x the fused pass stores)
     if (g->g1_nslice > 0)
-      k_sell_g1<T><<<slice_grid(g->g1_nslice), kBlock, 0, st>>>(g->g1_nslice, g->ncell, g->g1_row, g->g1_width,
-                                                               g->g1_sptr, g->g1_col, (const T*)g->g1_coef, p, ca);
+      k_sell_g1_split<T><<<grid_for(g->g1_nslice * kBlock), kBlock, 0, st>>>(g->g1_nslice, g->g1_row, g->g1_width,
+                                                                            g->g1_sptr, g->g1_col,
+                                                                            (const T*)g->g1_coef, p, ca);
     k_sg1v_t<T><<<gt, kBlock, 0, st>>>(g->ncell, L, ci, g->cell_g1v, (T)rou, (T)visc, (T)hdt, p, ca, cb, tpart);
     if (nc1 > 0)
       k_sell_v<T><<<n1, kBlock, 0, st>>>(g->v_nslice, g->ncell, g->v_row, g->v_width, g->v_sptr, g->v_col,
@@ -1219,7 +1277,8 @@ void g3(const pyib_grid* g, const pyib_work* ws, T* w, const double* scal, T* rr
   const int64_t ntile = (int64_t)gt.x * gt.y * gt.z, nc3 = g->g3_nslice > 0 ? n3 : 0;
   k_g3_m<T><<<(unsigned)(ntile + nc3), kBlock, 0, st>>>(ntile, gt, g->ncell, L, ci, cls ? g->edge_cls : g->edge_reg,
                                                        g->g3_nslice, g->g3_row, g->g3_width, g->g3_sptr, g->g3_col,
-                                                       (const T*)g->g3_coef, g->dir, nc3, cb, w, scal, rr, prec,
+                                                       (const T*)g->g3_coef, g->g3_mask, nc3, cb, w, scal, rr,
+                                                       prec, cls ? (const T*)g->g3_rprec : nullptr,
                                                        rr ? ws->partials + 2 * kRedGrid : nullptr,
                                                        rr ? ws->partials + 3 * kRedGrid : nullptr);
   if (nt) *nt = (int)ntile;

@@ -87,8 +87,6 @@ inline bool close(double a, double b, double tol, double scale) {
 // listed in the order of the NumPy construction; duplicates of one (row, col) are summed.
 struct Assembler {
   int ncomp;
-  std::vector<int32_t> row_pos;             // internal id -> position in sel (-1: not a row)
-  std::vector<int64_t> cnt;                 // entries per row
   std::vector<int32_t> ecol;
   std::vector<double> eval;
   std::vector<int32_t> erow;                // position of the entry's row
@@ -268,7 +266,7 @@ int build(const In& in, Built& b, std::string& err) {
     const double ref = 1.0 / std::pow(hc[c], 3.0);
     ok_vol[c] = close(in.vol_ci[c], ref, in.tol, ref);
   }
-  std::vector<uint8_t> rf1_ok(nface), rf2_ok(nface), links4(nface), v_face(nface);
+  std::vector<uint8_t> rf1_ok(nface), rf2_ok(nface), links4(nface), v_face(nface), b_face(nface);
   for (int64_t f = 0; f < nface; ++f) {
     const double hf = hc[in.f2c[2 * f]];
     const double* n = in.nf + 3 * f;
@@ -291,23 +289,47 @@ int build(const In& in, Built& b, std::string& err) {
     rf1_ok[f] = ok1;
     rf2_ok[f] = ok2;
     v_face[f] = same_h && close(std::fabs(in.aol[f]), hf, in.tol, hf);
+    b_face[f] = !m1 && close(in.aol[f], 2.0 * hf, in.tol, hf);   // boundary face, zero ghost, AoL = 2h
   }
   b.reg_g1.assign(in.ncell_i, 0);
-  std::vector<uint8_t> real(in.ncell_i, 0), v_geo(in.ncell_i, 0);
+  std::vector<uint8_t> real(in.ncell_i, 0), v_geo(in.ncell_i, 0), vb_geo(in.ncell_i, 0);
   for (int64_t c = 0; c < ncell; ++c) real[b.cell_map[c]] = 1;
   for (int64_t c = 0; c < ncell; ++c) {
     const int32_t s = in.cf_ptr[c], e = in.cf_ptr[c + 1], mid = in.cf_mid[c];
-    bool g1 = ok_vol[c] && (e - s) == 6, v = g1;
-    for (int32_t q = s; q < e && (g1 || v); ++q) {
+    bool g1 = ok_vol[c] && (e - s) == 6, v = g1, vb = g1;
+    for (int32_t q = s; q < e; ++q) {
       const int32_t f = in.cf_face[q];
       const bool first = q < mid;
       if (!((first ? rf1_ok[f] : rf2_ok[f]) && links4[f])) g1 = false;
       if (!v_face[f]) v = false;
+      if (!(in.cf_other[q] < ncell ? v_face[f] : b_face[f])) vb = false;
     }
     if (g1) b.reg_g1[b.cell_map[c]] = 1;
     if (v) v_geo[b.cell_map[c]] = 1;
+    if (vb) vb_geo[b.cell_map[c]] = 1;
   }
   for (int64_t c = 0; c < in.ncell_i; ++c) b.reg_cell[c] = b.reg_cell[c] && v_geo[c] && real[c];
+  // box-face V rows: cells of the coarsest level on the domain boundary, at least margin cells from
+  // the hole, whose faces inside the box are same-level faces with AoL = h and whose other faces are
+  // boundary faces with AoL = 2h (the stencil gives those 2 kA; reg_cell bit 1)
+  std::vector<uint8_t> reg_vb(in.ncell_i, 0);
+  {
+    const Lev& L = in.lev[N - 1];
+    const int64_t m = in.margin;
+    for (int64_t i = 0; i < L.n[0]; ++i)
+      for (int64_t j = 0; j < L.n[1]; ++j)
+        for (int64_t k = 0; k < L.n[2]; ++k) {
+          const bool onbox = i == 0 || i == L.n[0] - 1 || j == 0 || j == L.n[1] - 1 || k == 0 || k == L.n[2] - 1;
+          if (!onbox) continue;
+          if (L.hh[0] > L.hl[0] && i >= std::max<int64_t>(L.hl[0] - m, 0) && i < L.hh[0] + m &&
+              j >= std::max<int64_t>(L.hl[1] - m, 0) && j < L.hh[1] + m &&
+              k >= std::max<int64_t>(L.hl[2] - m, 0) && k < L.hh[2] + m)
+            continue;
+          const int64_t c = L.cell_off + (i * L.n[1] + j) * L.n[2] + k;
+          reg_vb[c] = vb_geo[c] && real[c] && !b.reg_cell[c];
+        }
+  }
+  for (int64_t c = 0; c < in.ncell_i; ++c) b.reg_cell[c] = b.reg_cell[c] || reg_vb[c];
   if (in.demote_neighbours) {
     std::vector<uint8_t> bad(in.ncell_i);
     for (int64_t c = 0; c < in.ncell_i; ++c) bad[c] = real[c] && !b.reg_cell[c];
@@ -320,6 +342,7 @@ int build(const In& in, Built& b, std::string& err) {
     for (int64_t c = 0; c < ncell; ++c)
       if (nb[c]) b.reg_cell[b.cell_map[c]] = 0;
   }
+  for (int64_t c = 0; c < in.ncell_i; ++c) b.reg_cell[c] = (uint8_t)((b.reg_cell[c] ? 1 : 0) | ((reg_vb[c] && b.reg_cell[c]) ? 2 : 0));
   // G3: four rot links on faces with closed-form rf1 and rf2, four real adjacent cells
   b.reg_edge.assign(in.nedge_i, 0);
   for (int64_t e = 0; e < nedge; ++e) {
@@ -348,7 +371,7 @@ int build(const In& in, Built& b, std::string& err) {
   for (int64_t e = 0; e < in.nedge_i; ++e)
     if (b.edge_inv[e] < 0) b.reg_edge[e] = 0;
   for (int64_t c = 0; c < in.ncell_i; ++c) {
-    b.stats[0] += b.reg_cell[c];
+    b.stats[0] += b.reg_cell[c] ? 1 : 0;
     b.stats[1] += b.reg_g1[c];
   }
   for (int64_t e = 0; e < in.nedge_i; ++e) b.stats[2] += b.reg_edge[e];

@@ -20,6 +20,9 @@ from .layout import (MAX_LEVELS, build_layout, composite_rows, dir_mask_internal
                      row_volumes, to_sell)
 
 
+G3_SIGMA = 64
+
+
 class HipGrid:
     def __init__(self, core, precisions=("float64",), margin=1, layout=None):
         self.core = core
@@ -41,7 +44,9 @@ class HipGrid:
         self._t = t
         self._comp = {}
         for name in ("g1", "vis", "g3"):
-            self._comp[name] = self._sell(getattr(lay, name))
+            # the G3 composite's rows gather and update edge vectors at the row: slices of 64
+            # consecutive rows (no length sort across slices) share their cache lines
+            self._comp[name] = self._sell(getattr(lay, name), G3_SIGMA if name == "g3" else None)
             if name == "vis":
                 self._comp[name]["rw"] = t(row_volumes(lay, core.geo, self._comp[name]["rows_host"]), torch.float64)
         self._coef = {}
@@ -50,9 +55,9 @@ class HipGrid:
         for prec in precisions:
             self.add_precision(prec)
 
-    def _sell(self, comp):
+    def _sell(self, comp, sigma=None):
         t = self._t
-        c = to_sell(comp)
+        c = to_sell(comp) if sigma is None else to_sell(comp, sigma=sigma)
         return {"rows": t(c.rows, torch.int32), "width": t(c.width, torch.int32), "sptr": t(c.sptr, torch.int64),
                 "col": t(c.col, torch.int32), "n": int(c.width.shape[0]), "coef64": c.coef, "rows_host": c.rows}
 
@@ -102,8 +107,10 @@ class HipGrid:
         cls = getattr(self, "_cls", None)
         if cls is not None and prec in cls["prec"]:
             g.edge_cls, g.cls_prec = ptr(self._edge_cls), ptr(cls["prec"][prec])
+            g.g3_rprec = ptr(cls["g3_rprec"][prec])
             for i in range(MAX_LEVELS):
                 g.prec_level[i], g.prec_level_g3[i] = float(cls["plev"][i]), float(cls["pg3"][i])
+        g.g3_mask = None if getattr(self, "_g3_dir_free", False) else ptr(self._dir)
         return g
 
     def set_dir_mask(self, dir_mask_ref):
@@ -114,7 +121,8 @@ class HipGrid:
         self._dir_host = m
         self._cls = None            # the preconditioner classes follow the mask: set them again
         g3 = self.lay.g3
-        self._comp["g3"] = self._sell(composite_rows(g3, np.flatnonzero(m[np.asarray(g3.rows)] == 0)))
+        self._comp["g3"] = self._sell(composite_rows(g3, np.flatnonzero(m[np.asarray(g3.rows)] == 0)), G3_SIGMA)
+        self._g3_dir_free = True
         for prec in self._struct:
             dt = torch.float64 if prec == "float64" else torch.float32
             self._coef[prec]["g3"] = torch.from_numpy(np.ascontiguousarray(self._comp["g3"]["coef64"])).to(
@@ -131,7 +139,14 @@ class HipGrid:
             raise RuntimeError("set_dir_mask comes before set_preconditioner")
         cls, plev, pg3 = edge_classes(self.lay, dir_host, prec_int_host)
         self._edge_cls = self._t(cls, torch.uint8)
-        self._cls = {"plev": plev, "pg3": pg3, "prec": dict(prec_tensors)}
+        # the G3 composite's rows read prec at their row from this per-lane copy (gathered from the
+        # same device arrays, so the values are those of prec[row])
+        rows = self._t(np.asarray(self._comp["g3"]["rows_host"], np.int64), torch.int64)
+        live = rows >= 0
+        rprec = {p: torch.where(live, t.index_select(0, rows.clamp(min=0)), torch.zeros((), dtype=t.dtype,
+                                                                                          device=t.device))
+                 for p, t in prec_tensors.items()}
+        self._cls = {"plev": plev, "pg3": pg3, "prec": dict(prec_tensors), "g3_rprec": rprec}
         for prec in self._struct:
             self._struct[prec] = self._make(prec)
 

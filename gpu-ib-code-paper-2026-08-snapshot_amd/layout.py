@@ -65,7 +65,7 @@ class StructuredLayout:
     edge_sigma: np.ndarray      # +-1
     edge_inv: np.ndarray        # internal -> reference edge (-1: empty slot)
     sigma_int: np.ndarray       # +-1 on internal slots (1 on empty)
-    reg_cell: np.ndarray        # uint8 over internal cells: V stencil row
+    reg_cell: np.ndarray        # uint8 over internal cells: bit 0 V stencil row, bit 1 box-face V row
     reg_edge: np.ndarray        # uint8 over internal edges: G3 stencil row
     reg_g1: np.ndarray = None   # uint8 over internal cells: G1 stencil row
     g1: Composite = None
@@ -223,7 +223,7 @@ def build_layout_numpy(mesh, geo, tables, margin=1, tol=1e-13, demote_neighbours
     lay = StructuredLayout(levels, ncell_i, nedge_i, cell_map, edge_map, sigma, edge_inv, sigma_int,
                            np.zeros(ncell_i, np.uint8), np.zeros(nedge_i, np.uint8))
     _classify(lay, margin)
-    _verify_regular(lay, mesh, geo, tables, tol, demote_neighbours)
+    _verify_regular(lay, mesh, geo, tables, tol, demote_neighbours, margin)
     _composites(lay, geo, tables)
     return lay
 
@@ -231,6 +231,16 @@ def build_layout_numpy(mesh, geo, tables, margin=1, tol=1e-13, demote_neighbours
 def _reg_cells_level(lv, m):
     r = np.zeros((lv.nx, lv.ny, lv.nz), bool)
     r[m:lv.nx - m, m:lv.ny - m, m:lv.nz - m] = True
+    a, b = lv.hole_lo, lv.hole_hi
+    if b[0] > a[0]:
+        r[max(a[0] - m, 0):b[0] + m, max(a[1] - m, 0):b[1] + m, max(a[2] - m, 0):b[2] + m] = False
+    return r
+
+
+def _box_face_cells(lv, m):
+    """Cells on the level box's boundary that lie at least m cells from the hole."""
+    r = np.ones((lv.nx, lv.ny, lv.nz), bool)
+    r[1:lv.nx - 1, 1:lv.ny - 1, 1:lv.nz - 1] = False
     a, b = lv.hole_lo, lv.hole_hi
     if b[0] > a[0]:
         r[max(a[0] - m, 0):b[0] + m, max(a[1] - m, 0):b[1] + m, max(a[2] - m, 0):b[2] + m] = False
@@ -271,7 +281,7 @@ def _close(a, b, tol, scale):
     return np.abs(a - b) <= tol * scale if tol > 0 else a == b
 
 
-def _verify_regular(lay, mesh, geo, tables, tol, demote_neighbours=True):
+def _verify_regular(lay, mesh, geo, tables, tol, demote_neighbours=True, margin=1):
     """Regularity per pass, from the mesh geometry (each pass only needs its own data):
 
     * G1 (cell <- edges): closed-form vol_ci, exactly six faces, and on each face this
@@ -319,6 +329,19 @@ def _verify_regular(lay, mesh, geo, tables, tol, demote_neighbours=True):
     v_geo = np.zeros(lay.ncell, bool)
     v_geo[lay.cell_map[v_ok]] = True
     reg_v &= v_geo & real
+    # box-face V rows: cells of the coarsest level on the domain boundary (at least margin cells from
+    # the hole) whose faces inside the box are same-level faces with AoL = h and whose other faces are
+    # boundary faces (zero ghost) with AoL = 2h; the stencil gives those faces 2 kA (reg_cell bit 1)
+    lc = lay.levels[-1]
+    bface = ~np.asarray(geo.m1) & _close(geo.AoL, 2.0 * hf, tol, hf)
+    vb_ent = np.where(tables.cf_other < ncell, v_face[f], bface[f])
+    vb_ok = ok_vol & (cnt == 6) & (np.bincount(own, weights=(~vb_ent).astype(np.float64), minlength=ncell) == 0)
+    reg_vb = np.zeros(lay.ncell, bool)
+    reg_vb[lay.cell_map[vb_ok]] = True
+    box = np.zeros(lay.ncell, bool)
+    box[lc.cell_off:lc.cell_off + lc.ncell] = _box_face_cells(lc, margin).ravel()
+    reg_vb &= box & real & ~reg_v
+    reg_v |= reg_vb
     if demote_neighbours:
         bad = real & ~reg_v
         nb = np.zeros(ncell, bool)
@@ -328,8 +351,9 @@ def _verify_regular(lay, mesh, geo, tables, tol, demote_neighbours=True):
         badr = bad[lay.cell_map]
         nb[own[inner][badr[tables.cf_other[inner]]]] = True
         reg_v[lay.cell_map[nb]] = False
+        reg_vb &= reg_v
     lay.reg_g1 = reg_g1.astype(np.uint8)
-    lay.reg_cell = reg_v.astype(np.uint8)
+    lay.reg_cell = reg_v.astype(np.uint8) | (reg_vb.astype(np.uint8) << 1)
     # G3: structural (off the box boundary, four real adjacent cells) and geometric
     ecnt = np.diff(tables.rot_ptr)
     eown = np.repeat(np.arange(geo.nedge), ecnt)
@@ -503,18 +527,21 @@ def g1v_flags(lay):
     """Per internal cell, the flags of the fused G1+V pass: bit 0 = regular G1 row (x =
     interp curl p formed by the stencil), bit 1 = that x must also be stored, because a
     composite V row reads it (as its own row or a column), bit 2 = regular V row, bit 3 = x
-    comes from memory (a real cell whose G1 row is composite: the composite G1 pass wrote it).
+    comes from memory (a real cell whose G1 row is composite: the composite G1 pass wrote it),
+    bit 4 = a box-face V row (its faces that leave the box take twice the stencil coefficient).
     The pass forms x in LDS, stores only the bit-1 cells, and takes x = 0 on the cells with
     neither bit 0 nor bit 3 (the hole, whose x no stencil row reads)."""
     keep = np.zeros(lay.ncell, bool)
     keep[np.asarray(lay.vis.rows, dtype=np.int64)] = True
     keep[np.asarray(lay.vis.col, dtype=np.int64)] = True
     reg = lay.reg_g1.astype(bool)
-    regv = lay.reg_cell.astype(bool)
+    regv = (lay.reg_cell & 1).astype(bool)
+    boxv = (lay.reg_cell & 2).astype(bool)
     real = np.zeros(lay.ncell, bool)
     real[np.asarray(lay.cell_map, dtype=np.int64)] = True
     return (reg.astype(np.uint8) | ((reg & keep).astype(np.uint8) << 1)
-            | (regv.astype(np.uint8) << 2) | ((real & ~reg).astype(np.uint8) << 3)).astype(np.uint8)
+            | (regv.astype(np.uint8) << 2) | ((real & ~reg).astype(np.uint8) << 3)
+            | (boxv.astype(np.uint8) << 4)).astype(np.uint8)
 
 
 def row_volumes(lay, geo, rows):

@@ -105,7 +105,9 @@ typedef struct pyib_grid {
   int32_t nlevels;
   int64_t ncell, nedge;     /* internal (padded) sizes */
   pyib_level lev[PYIB_MAX_LEVELS];
-  const uint8_t *cell_reg;  /* ncell: V row computed by the stencil */
+  const uint8_t *cell_reg;  /* ncell: V row computed by the stencil (bit 0); bit 1: a box-face row of
+                             * the coarsest level, whose faces on the box take twice the coefficient
+                             * (zero ghost, AoL = 2h) */
   const uint8_t *cell_reg_g1; /* ncell: G1 row computed by the stencil */
   const uint8_t *edge_reg;  /* nedge: G3 row computed by the stencil (w must be zero-initialised:
                              * empty slots, and Dirichlet rows left out of the G3 composite, are
@@ -113,8 +115,9 @@ typedef struct pyib_grid {
   const uint8_t *dir;       /* nedge: Dirichlet edge or empty slot (residual forced to 0) */
   const uint8_t *cell_g1v;  /* ncell, flags of the fused G1+V pass: bit 0 = cell_reg_g1, bit 1 = its
                              * x = interp(curl p) is stored for the composite V rows (the pass keeps
-                             * the rest in LDS), bit 2 = cell_reg, bit 3 = x is read from memory
-                             * (a real cell with a composite G1 row); x = 0 on the other cells */
+                             * the rest in LDS), bit 2 = cell_reg bit 0, bit 3 = x is read from memory
+                             * (a real cell with a composite G1 row), bit 4 = cell_reg bit 1; x = 0 on
+                             * the other cells */
   /* composite rows of the other (interface/boundary) rows in sliced ELL: slices of 64
    * rows (one wavefront), entry j of lane l of slice s at sptr[s] + 64*j + l, width[s]
    * entries per lane, row[64*s + l] = internal row id or -1 (padding, col 0, coef 0) */
@@ -136,6 +139,11 @@ typedef struct pyib_grid {
   double prec_level[PYIB_MAX_LEVELS];     /* prec on the bit-2 slots of each level */
   double prec_level_g3[PYIB_MAX_LEVELS];  /* prec_level when every regular G3 row of the level is
                                            * bit 2, else 0 (then the G3 pass reads prec) */
+  const uint8_t *g3_mask;   /* Dirichlet mask the G3 composite applies to its rows, or NULL when its
+                             * rows hold no Dirichlet row (after the mask is known) */
+  const void *g3_rprec;     /* 64*g3_nslice, this dtype: cls_prec at each lane's G3 composite row (0 on
+                             * padding), read instead of gathering prec[row] when the PCG's prec is
+                             * cls_prec; or NULL */
 } pyib_grid;
 
 const char *pyib_last_error(void);

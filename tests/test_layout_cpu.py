@@ -23,6 +23,12 @@ def blocks(lay, lv, v):
     return [v[lv.e_off[d]:lv.e_off[d] + int(np.prod(shp[d]))].reshape(shp[d]) for d in range(3)]
 
 
+def box_faces(lv):
+    """Number of each cell's faces on the level box's boundary."""
+    i, j, k = np.meshgrid(np.arange(lv.nx), np.arange(lv.ny), np.arange(lv.nz), indexing="ij")
+    return ((i == 0).astype(int) + (i == lv.nx - 1) + (j == 0) + (j == lv.ny - 1) + (k == 0) + (k == lv.nz - 1))
+
+
 def emulate(lay, p, rou, visc, hdt, cells=False):
     """Structured stencils on regular rows + composite rows elsewhere (the kernels' arithmetic);
     with ``cells`` also the intermediate cell vectors x = interp curl p and V x."""
@@ -46,12 +52,15 @@ def emulate(lay, p, rou, visc, hdt, cells=False):
     for lv in lay.levels:
         sl = slice(lv.cell_off, lv.cell_off + lv.ncell)
         reg = lay.reg_cell[sl].astype(bool)
+        nbox = box_faces(lv) * ((lay.reg_cell[sl] & 2) != 0).reshape(lv.nx, lv.ny, lv.nz)
         k = hdt * visc / (lv.h * lv.h)
         for d in range(3):
             x = np.zeros((lv.nx + 2, lv.ny + 2, lv.nz + 2))
             x[1:-1, 1:-1, 1:-1] = vc[d, sl].reshape(lv.nx, lv.ny, lv.nz)
             lap = (x[2:, 1:-1, 1:-1] + x[:-2, 1:-1, 1:-1] + x[1:-1, 2:, 1:-1] + x[1:-1, :-2, 1:-1]
                    + x[1:-1, 1:-1, 2:] + x[1:-1, 1:-1, :-2] - 6 * x[1:-1, 1:-1, 1:-1])
+            # a face that leaves the box (zero ghost) has AoL = 2h: twice the coefficient
+            lap = lap - nbox * x[1:-1, 1:-1, 1:-1]
             y = rou * x[1:-1, 1:-1, 1:-1] - k * lap
             v2[d, sl][reg] = y.ravel()[reg]
     vi = lay.vis
@@ -172,7 +181,8 @@ def test_g1v_flags_cover_the_composite_v_reads():
     lay = build_layout(m, g, build_tables(g))
     f = g1v_flags(lay)
     assert np.array_equal(f & 1, lay.reg_g1)
-    assert np.array_equal((f >> 2) & 1, lay.reg_cell)
+    assert np.array_equal((f >> 2) & 1, lay.reg_cell & 1)
+    assert np.array_equal((f >> 4) & 1, (lay.reg_cell >> 1) & 1)
     assert not np.any((f & 2).astype(bool) & ~(f & 1).astype(bool))
     read = np.unique(np.concatenate([lay.vis.rows, lay.vis.col]))
     ok = ((f[read] & 2) != 0) | (lay.reg_g1[read] == 0)
@@ -241,11 +251,15 @@ def _tile(lay, lv, p, flags, xmem, y, j0, k0, i0, i1, rou, visc, hdt):
             j, k = j0 + tj, k0 + tk
             ok = (j < ny) & (k < nz)
             cc = co + (i * ny + j) * nz + k
-            rv = ok & ((flags[np.where(ok, cc, 0)] & 4) != 0)
+            fl = flags[np.where(ok, cc, 0)]
+            rv = ok & ((fl & 4) != 0)
+            nbox = np.where((fl & 16) != 0, ((i == 0) + (i == nx - 1) + (j == 0) + (j == ny - 1) + (k == 0)
+                                              + (k == nz - 1)).astype(float), 0.0)
             s0 = slot(i)
             for d in range(3):
                 xc = sc[s0][d][qo]
                 aa = kA * (sc[a][d][qo] - xc) + kA * (sc[s0][d][qo + RK] - xc) + kA * (sc[s0][d][qo + 1] - xc)
+                aa = aa - nbox * (kA * xc)
                 bb = kA * (xc - xm[d]) + kA * (xc - sc[s0][d][qo - RK]) + kA * (xc - sc[s0][d][qo - 1])
                 y[3 * cc[rv] + d] = (rou * xc - hv * (aa - bb))[rv]
         if ip >= i0:
