@@ -390,6 +390,15 @@ struct Prefetch {
   T v[R];
 };
 
+// Nontemporal store.  Measured at D32: the float32 G1 pass's x and the float32 V pass's y go out
+// nontemporally (each lowered the float32 iteration by about 6 us); in float64 the fused pass's
+// stores stay cached (nontemporal y made it 225 -> 242 us), and so do the G3 pass's r loads and
+// stores in both precisions (nontemporal made it 263 -> 337 us in float64).
+template <typename T>
+__device__ __forceinline__ void st_nt(T* p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
 // ---- G1: vc = interp(curl p) on the cells of plane i (reads x-edges of plane i, y/z-edges
 // of planes i and i+1)
 template <typename T>
@@ -481,9 +490,9 @@ __device__ __forceinline__ void sg1_tile(int64_t lb, const dim3& tg, int64_t nce
       const T uz0 = (PY(b, 0) - PY(a, 0)) - (PX(1, 0) - PX(0, 0));
       const T uz1 = (PY(b, 1) - PY(a, 1)) - (PX(1, 1) - PX(0, 1));
       const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
-      vc[3 * c] = (ux0 + ux1) * cf;
-      vc[3 * c + 1] = (uy0 + uy1) * cf;
-      vc[3 * c + 2] = (uz0 + uz1) * cf;
+      st_nt(vc + 3 * c, (ux0 + ux1) * cf);
+      st_nt(vc + 3 * c + 1, (uy0 + uy1) * cf);
+      st_nt(vc + 3 * c + 2, (uz0 + uz1) * cf);
     }
     __syncthreads();
   };
@@ -781,7 +790,7 @@ __device__ __forceinline__ void sv_tile(int64_t lb, const dim3& tg, int64_t ncel
           T b = kA * (xc - sx[sm][d][q]) + kA * (xc - sx[s0][d][q - PK]);
           b = b + kA * (xc - sx[s0][d][q - 1]);
           const T yv = rou * xc - hv * (a - b);
-          y[3 * c + d] = yv;
+          st_nt(y + 3 * c + d, yv);
           dot += (double)xc * (double)yv;
         }
       }
@@ -954,14 +963,20 @@ __device__ __forceinline__ void sg3_tile(int64_t lb, const dim3& tg, int64_t nce
   }
 }
 
-// ---- one launch per pass: the tiled stencil blocks [0, ntile) and, after them, the composite-row
-// blocks [ntile, ntile + ncomp).  Both write disjoint rows from the same inputs, so the composite
-// blocks fill the CUs the last tiles leave idle instead of waiting for the whole tiled pass.  The
-// tile blocks keep their dispatch index (XCD placement and partial slots are those of a launch of
-// the tile grid alone) and the composite blocks their index among themselves, so every sum is
-// formed in the same order as by two launches.  The composite rows gather in batches of
+// ---- one launch per pass: the tiled stencil blocks and the composite-row blocks (after the tiles,
+// or before them for the float64 G3 pass).  Both write disjoint rows from the same inputs, so the
+// composite blocks share the CUs with the tiles instead of waiting for the whole tiled pass.  Each
+// tile block keeps its XCD and its partial slot (its index among the tile blocks), and each
+// composite block its index among themselves, so every sum is formed in the same order as by two
+// launches.  The composite rows gather in batches of
 // kMergedBatch, which keeps the merged kernel's registers near the tiled pass's own.
 constexpr int kMergedBatch = 4;
+// the G3 launch places its composite blocks before the tiles in float64 (measured at D32: 267
+// against 278 us) and after them in float32 (163 against 167 us)
+template <typename T>
+struct G3CompFirst {
+  static constexpr bool value = sizeof(T) == 8;
+};
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_g1_m(int64_t ntile, dim3 tg, int64_t ncell, Levels L, int ci,
@@ -1001,11 +1016,21 @@ __global__ __launch_bounds__(256) void k_g3_m(int64_t ntile, dim3 tg, int64_t nc
                                               const double* __restrict__ scal, T* __restrict__ rr,
                                               const T* __restrict__ prec, const T* __restrict__ rprec,
                                               double* __restrict__ tpart, double* __restrict__ cpart) {
-  if (blockIdx.x < ntile)
-    sg3_tile<T>(blockIdx.x, tg, ncell, L, ci, reg, y, w, scal, rr, prec, tpart);
-  else
-    sell_g3_rows<T, kMergedBatch>(blockIdx.x - ntile, ncomp, nslice, row, width, sptr, col, coef, mask, y, w, scal,
-                                  rr, prec, rprec, cpart);
+  if constexpr (G3CompFirst<T>::value) {
+    // composite blocks first, padded to a multiple of 8 blocks so each tile block keeps its XCD
+    const int64_t nf = (ncomp + 7) & ~int64_t(7);
+    if (blockIdx.x >= nf)
+      sg3_tile<T>(blockIdx.x - nf, tg, ncell, L, ci, reg, y, w, scal, rr, prec, tpart);
+    else if (blockIdx.x < ncomp)
+      sell_g3_rows<T, kMergedBatch>(blockIdx.x, ncomp, nslice, row, width, sptr, col, coef, mask, y, w, scal, rr,
+                                    prec, rprec, cpart);
+  } else {
+    if (blockIdx.x < ntile)
+      sg3_tile<T>(blockIdx.x, tg, ncell, L, ci, reg, y, w, scal, rr, prec, tpart);
+    else
+      sell_g3_rows<T, kMergedBatch>(blockIdx.x - ntile, ncomp, nslice, row, width, sptr, col, coef, mask, y, w, scal,
+                                    rr, prec, rprec, cpart);
+  }
 }
 
 // sum of two partial sets (nb1 from the composite rows, nb2 from the stencil rows)
@@ -1069,7 +1094,11 @@ __global__ void k_gcg_dir_x(int64_t n, const double* __restrict__ scal, const T*
 // says it holds: the empty slots and Dirichlet rows keep p = 0 and x as they are.  A float
 // thread takes a 16-byte group of 4 slots (one 4-byte class load, then one 16-byte load per
 // vector); a double thread takes one slot (measured at D32: 312 us against 350 us for 2-slot
-// groups, both over the same bytes).  A group of dead slots is not touched at all.
+// groups, both over the same bytes).  A group of dead slots is not touched at all.  The class,
+// r, p and x streams are loaded and stored nontemporally: each is touched once per iteration
+// here, and streaming them past the caches took this kernel from 332 to 285 us at D32 (float64)
+// and sped up the passes that follow it, which no longer meet its dirty lines (the float64
+// iteration went from 928 to 849 us, the float32 one from 564 to 520 us).
 template <typename T>
 struct DirGroup {
   static constexpr int W = sizeof(T) == 4 ? 4 : 1;
@@ -1078,6 +1107,14 @@ struct DirGroup {
     T v[W];
   };
   static constexpr Cls kLive = W == 4 ? (Cls)0x02020202u : (Cls)0x02u;
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  using NV = typename std::conditional<W == 4, f4, T>::type;   // one load / store per group
+  static __device__ __forceinline__ T get(const NV& v, int u) {
+    if constexpr (W == 4) return v[u]; else return v;
+  }
+  static __device__ __forceinline__ void set(NV& v, int u, T a) {
+    if constexpr (W == 4) v[u] = a; else v = a;
+  }
 };
 template <typename T>
 __device__ __forceinline__ T level_prec(const PrecTable& pt, int c) {
@@ -1099,11 +1136,15 @@ __global__ void k_gcg_dir_x_cls(int64_t n, const double* __restrict__ scal, cons
   const int64_t nq = n / W, stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   for (int64_t q = t0; q < nq; q += stride) {
-    const uint32_t cw = reinterpret_cast<const typename VT::Cls*>(cls)[q];
+    const uint32_t cw = __builtin_nontemporal_load(reinterpret_cast<const typename VT::Cls*>(cls) + q);
     if (!(cw & VT::kLive)) continue;
-    const V pv = reinterpret_cast<const V*>(p)[q];
-    const V rv = reinterpret_cast<const V*>(r)[q];
-    V xv = reinterpret_cast<const V*>(x)[q];
+    using NV = typename VT::NV;
+    const NV pn = __builtin_nontemporal_load(reinterpret_cast<const NV*>(p) + q);
+    const NV rn = __builtin_nontemporal_load(reinterpret_cast<const NV*>(r) + q);
+    const NV xn = __builtin_nontemporal_load(reinterpret_cast<const NV*>(x) + q);
+    V pv, rv, xv;
+#pragma unroll
+    for (int u = 0; u < W; ++u) { pv.v[u] = VT::get(pn, u); rv.v[u] = VT::get(rn, u); xv.v[u] = VT::get(xn, u); }
     V po = pv;
 #pragma unroll
     for (int u = 0; u < W; ++u) {
@@ -1114,8 +1155,11 @@ __global__ void k_gcg_dir_x_cls(int64_t n, const double* __restrict__ scal, cons
         po.v[u] = rv.v[u] * m + beta * pv.v[u];
       }
     }
-    reinterpret_cast<V*>(x)[q] = xv;
-    reinterpret_cast<V*>(p)[q] = po;
+    NV xo, pw;
+#pragma unroll
+    for (int u = 0; u < W; ++u) { VT::set(xo, u, xv.v[u]); VT::set(pw, u, po.v[u]); }
+    __builtin_nontemporal_store(xo, reinterpret_cast<NV*>(x) + q);
+    __builtin_nontemporal_store(pw, reinterpret_cast<NV*>(p) + q);
   }
   for (int64_t i = W * nq + t0; i < n; i += stride) {      // the last n % W slots
     const int c = cls[i];
@@ -1275,7 +1319,8 @@ void g3(const pyib_grid* g, const pyib_work* ws, T* w, const double* scal, T* rr
   const dim3 gt = tile_grid(L, &ci);
   const int n3 = slice_red_grid(g->g3_nslice);
   const int64_t ntile = (int64_t)gt.x * gt.y * gt.z, nc3 = g->g3_nslice > 0 ? n3 : 0;
-  k_g3_m<T><<<(unsigned)(ntile + nc3), kBlock, 0, st>>>(ntile, gt, g->ncell, L, ci, cls ? g->edge_cls : g->edge_reg,
+  const int64_t nc3_grid = G3CompFirst<T>::value ? ((nc3 + 7) & ~int64_t(7)) : nc3;
+  k_g3_m<T><<<(unsigned)(ntile + nc3_grid), kBlock, 0, st>>>(ntile, gt, g->ncell, L, ci, cls ? g->edge_cls : g->edge_reg,
                                                        g->g3_nslice, g->g3_row, g->g3_width, g->g3_sptr, g->g3_col,
                                                        (const T*)g->g3_coef, g->g3_mask, nc3, cb, w, scal, rr,
                                                        prec, cls ? (const T*)g->g3_rprec : nullptr,
