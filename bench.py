@@ -354,6 +354,70 @@ def kernels_only(n_iter, precision, log):
     return 0
 
 
+def period_harmonics(d, steps_per_period, log):
+    """Mixed precision against float64 over one full oscillation period of the oscillating sphere
+    at cells_per_diameter d (3 nested levels, finest 8D x 6D x 6D, Re0 = 0.2, A = 0.1 D, dt =
+    T/steps_per_period), from rest: the least-squares harmonic vector (a, b, c) of body_force_x,
+    a cos(wt + phase) + b sin(wt + phase) + c over the period (the reference's validation fit,
+    tools/check_pressure_validation.py:32-38), for each precision, and their relative difference."""
+    import numpy as np
+    import torch
+    from pyib_hip import case as HC
+
+    f = 3.6 / (2.0 * math.pi)
+    T = 1.0 / f
+    out = {}
+    mesh = None
+    for prec in ("float64", "mixed"):
+        cfg = headline_config(steps_per_period, prec)
+        cfg["name"] = f"period_d{d}"
+        cfg["mesh"] = {"kind": "nested_box", "cells_per_diameter": d, "half_extents_d": [4.0, 3.0, 3.0], "levels": 3}
+        cfg["ib"]["marker_spacing"] = 0.5 / d
+        cfg["time"]["dt"] = T / steps_per_period
+        cfg = HC.validate_config(cfg)
+        if mesh is None:
+            mesh, _, _ = HC.build_mesh(cfg)
+        rt = HC.build_runtime(cfg, mesh=mesh)
+        drv = rt.driver
+        s, s_old, vc = HC.initial_state(rt)
+        times, fx, iters = [], [], 0
+        t0 = time.perf_counter()
+        for step in range(1, steps_per_period + 1):
+            drv.time = (step - 1) * rt.dt
+            s, vc, s_old = drv.step(s, s_old, rt.dt, rt.dt, inner=rt.inner_iterations)
+            drv.time = step * rt.dt
+            load = rt.load_correction.advance(vc, HC.sphere_center(cfg, drv.time), rt.dt, drv.ib_force_E)
+            times.append(drv.time)
+            fx.append(float(load.body_force[0]))
+            iters += drv.helmholtz_iterations_sum_last_step
+            if step % 20 == 0:
+                log(f"[period] {prec} step {step}/{steps_per_period}: Fx {fx[-1]:.12e}, "
+                    f"{time.perf_counter() - t0:.0f} s")
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        omega, phase = 2.0 * math.pi * f, float(cfg["ib"]["motion"]["phase"])
+        ang = omega * np.asarray(times) + phase
+        A = np.column_stack([np.cos(ang), np.sin(ang), np.ones_like(ang)])
+        coeff = np.linalg.lstsq(A, np.asarray(fx), rcond=None)[0]
+        out[prec] = {"harmonic_abc": [float(c) for c in coeff], "wall_s": wall, "cg_iterations": int(iters),
+                     "body_force_x": fx}
+        out["mesh"] = {"cells": int(rt.core.ncell), "edges": int(rt.core.nedge),
+                       "markers": int(rt.reference_markers.shape[0])}
+        del rt, drv, s, s_old, vc
+        torch.cuda.empty_cache()
+    h64, hmx = np.asarray(out["float64"]["harmonic_abc"]), np.asarray(out["mixed"]["harmonic_abc"])
+    f64, fmx = np.asarray(out["float64"]["body_force_x"]), np.asarray(out["mixed"]["body_force_x"])
+    res = {"case": f"oscillating sphere D{d}, 3 nested levels, finest {8 * d}x{6 * d}x{6 * d}, one period "
+                   f"from rest in {steps_per_period} steps", "mesh": out["mesh"],
+           "float64": {k: v for k, v in out["float64"].items() if k != "body_force_x"},
+           "mixed": {k: v for k, v in out["mixed"].items() if k != "body_force_x"},
+           "harmonic_relerr": float(np.linalg.norm(hmx - h64) / np.linalg.norm(h64)),
+           "force_history_max_relerr": float(np.max(np.abs(fmx - f64)) / np.max(np.abs(f64))),
+           "mixed_speedup": out["float64"]["wall_s"] / out["mixed"]["wall_s"]}
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def _kernel_label(name):
     """`pyib::k_name<T>` (or the name itself) from a demangled kernel signature."""
     import re
@@ -465,6 +529,10 @@ def main(argv=None):
                         "WRITE_SIZE databases); --kernels-only gives their iteration count, --precision theirs")
     p.add_argument("--profile-command", action="append", default=[],
                    help="a command line to quote in the summary (repeatable)")
+    p.add_argument("--period-harmonics", type=int, default=0, metavar="D",
+                   help="mixed against float64 over one oscillation period at cells_per_diameter D "
+                        "(finest 8Dx6Dx6D, 3 levels)")
+    p.add_argument("--period-steps", type=int, default=200, help="steps per period of --period-harmonics")
     p.add_argument("--mixed-inner-tol", type=float, default=None,
                    help="float32 residual reduction between float64 reliable updates (default: the schema's)")
     a = p.parse_args(argv)
@@ -481,6 +549,8 @@ def main(argv=None):
         n_it = a.kernels_only or 100
         print(summarize_profile(*a.summarize_profile, n_it, a.precision, 40986624, a.profile_command), end="")
         return 0
+    if a.period_harmonics:
+        return period_harmonics(a.period_harmonics, a.period_steps, log)
     if a.kernels_only:
         return kernels_only(a.kernels_only, "mixed" if a.precision == "mixed" else "float64", log)
     precs = ["float64", "mixed"] if a.precision == "both" else [a.precision]

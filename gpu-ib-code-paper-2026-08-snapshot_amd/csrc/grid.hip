@@ -12,12 +12,14 @@
 //
 //   G1  vc_d(c)  = (U_d(lo) + U_d(hi)) / (2h^2)       U = discrete curl of p
 //   V   y(c)     = rou x - (hdt/h^3)(a - b),  a = sum_up (h visc)(x_n - x), b = sum_dn (h visc)(x - x_n)
+//                  (a face of the coarsest level's box: x_n = 0 and twice the coefficient)
 //   G3  w_e      = sum over the edge's 4 faces of +-(h/2)(y(c1) + y(c2))
 //
 // Inside a PCG iteration the passes also carry the CG's vector work: the V pass forms
 // p.A p from the cells (sum vol x.(V x)), so alpha is known before G3, which then applies
 // r -= alpha w without storing w; x += alpha p rides on the direction update.  In float64
-// G1 and V run as one pass (x stays in LDS).
+// G1 and V run as one pass (x stays in LDS).  Where the data dependences allow, a pass's
+// tiles and its composite rows share one launch.
 #include "common.h"
 
 #include <type_traits>

@@ -21,6 +21,7 @@ def test_measured_traffic_comes_from_the_profile():
     for prec, alg in (("float64", 104.0), ("float32", 52.0)):
         t = bench.measured_traffic(prec)
         assert t is not None
-        # the fused iteration moves more than the textbook 13 words (cell intermediates,
-        # composite rows) but well under twice as much
-        assert alg * 40986624 < t < 2 * alg * 40986624
+        # the fused iteration moves about the textbook 13 words: float64 a little under them (the
+        # G1+V pass keeps its cell intermediate on chip), float32 more (the cell intermediates and
+        # composite rows in float32 weigh more against the halved edge words), but well under 1.5x
+        assert 0.9 * alg * 40986624 < t < 1.5 * alg * 40986624
