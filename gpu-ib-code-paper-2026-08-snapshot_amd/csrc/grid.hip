@@ -314,6 +314,15 @@ __device__ __forceinline__ void sell_g3_rows(int64_t blk, int64_t nblk, int64_t 
 #ifndef PYIB_TILE_CI3
 #define PYIB_TILE_CI3 8
 #endif
+#ifndef PYIB_TILE_PL32
+#define PYIB_TILE_PL32 4
+#endif
+#ifndef PYIB_G3_PL64
+#define PYIB_G3_PL64 2
+#endif
+#ifndef PYIB_G3_PL32
+#define PYIB_G3_PL32 2
+#endif
 constexpr int TJ = 8, TK = 32, CI = PYIB_TILE_CI, CI3 = PYIB_TILE_CI3;   // 256 threads: tj = tid / TK, tk = tid % TK
 
 // grid of the tiled passes; *ci = planes per block (given, doubled until the block count fits
@@ -403,7 +412,16 @@ __device__ __forceinline__ void st_nt(T* p, T v) {
 
 // ---- G1: vc = interp(curl p) on the cells of plane i (reads x-edges of plane i, y/z-edges
 // of planes i and i+1)
+// Planes per step of the float32 G1 and V tile passes.  A step waits once for the loads issued
+// in the step before (stores share the load counter, so the compiler cannot keep two steps in
+// flight); with a float32 plane of about 4 KB per block, four planes per step are what keep
+// enough bytes in flight per CU for HBM.
 template <typename T>
+struct TilePlanes {
+  static constexpr int value = sizeof(T) == 4 ? PYIB_TILE_PL32 : 1;
+};
+
+template <typename T, int PL>
 __device__ __forceinline__ void sg1_tile(int64_t lb, const dim3& tg, int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
                                                const T* __restrict__ p, T* __restrict__ vc) {
   int bx, by, bz;
@@ -415,95 +433,114 @@ __device__ __forceinline__ void sg1_tile(int64_t lb, const dim3& tg, int64_t nce
   constexpr int YJ = TJ, YK = TK + 1;           // y-edges: k..k+1
   constexpr int ZJ = TJ + 1, ZK = TK;           // z-edges: j..j+1
   constexpr int R = 2;                          // every staged plane has < 2*256 entries
-  __shared__ T sx[2][XJ * XK];
-  __shared__ T sy[2][YJ * YK];
-  __shared__ T sz[2][ZJ * ZK];
+  constexpr int NS = PL + 1;                    // y/z-edge planes i..i+PL of a step
+  __shared__ T sx[PL][XJ * XK];
+  __shared__ T sy[NS][YJ * YK];
+  __shared__ T sz[NS][ZJ * ZK];
   const int ny1 = v.ny + 1, nz1 = v.nz + 1;
-  // loads of step i: x-edges of plane i, y/z-edges of plane i+1
-  auto fetch = [&](int i, Prefetch<T, R>& fx, Prefetch<T, R>& fy, Prefetch<T, R>& fz) {
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int q = threadIdx.x + r * 256;
-      {
-        const int j = t.j0 + q / XK, k = t.k0 + q % XK;
-        const bool in = q < XJ * XK && i < v.nx && j < ny1 && k < nz1;
-        fx.v[r] = in ? p[v.e0 + ((int64_t)i * ny1 + j) * nz1 + k] : T(0);
-      }
-      {
-        const int j = t.j0 + q / YK, k = t.k0 + q % YK;
-        const bool in = q < YJ * YK && i + 1 <= v.nx && j < v.ny && k < nz1;
-        fy.v[r] = in ? p[v.e1 + ((int64_t)(i + 1) * v.ny + j) * nz1 + k] : T(0);
-      }
-      {
-        const int j = t.j0 + q / ZK, k = t.k0 + q % ZK;
-        const bool in = q < ZJ * ZK && i + 1 <= v.nx && j < ny1 && k < v.nz;
-        fz.v[r] = in ? p[v.e2 + ((int64_t)(i + 1) * ny1 + j) * v.nz + k] : T(0);
-      }
-    }
-  };
-  auto store = [&](int i, const Prefetch<T, R>& fx, const Prefetch<T, R>& fy, const Prefetch<T, R>& fz) {
-    const int a = i & 1, b = a ^ 1;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int q = threadIdx.x + r * 256;
-      if (q < XJ * XK) sx[a][q] = fx.v[r];
-      if (q < YJ * YK) sy[b][q] = fy.v[r];
-      if (q < ZJ * ZK) sz[b][q] = fz.v[r];
-    }
-  };
-  // y/z-edges of plane i0 (step i0 reads them from slot i0 & 1)
-  for (int q = threadIdx.x; q < YJ * YK; q += blockDim.x) {
-    const int j = t.j0 + q / YK, k = t.k0 + q % YK;
-    const bool in = t.i0 <= v.nx && j < v.ny && k < nz1;
-    sy[t.i0 & 1][q] = in ? p[v.e1 + ((int64_t)t.i0 * v.ny + j) * nz1 + k] : T(0);
-  }
-  for (int q = threadIdx.x; q < ZJ * ZK; q += blockDim.x) {
-    const int j = t.j0 + q / ZK, k = t.k0 + q % ZK;
-    const bool in = t.i0 <= v.nx && j < ny1 && k < v.nz;
-    sz[t.i0 & 1][q] = in ? p[v.e2 + ((int64_t)t.i0 * ny1 + j) * v.nz + k] : T(0);
-  }
-  // two planes in flight: buffer A serves the even steps from i0, B the odd ones; each step
-  // stores its plane, then reloads its buffer two planes ahead (with the cell's flag)
-  struct Pre {
-    Prefetch<T, R> x, y, z;
-    uint8_t g;
-  };
+  // 32-bit offsets inside a level's blocks (each has < 2^31 slots) from per-level bases
+  const T* __restrict__ pe0 = p + v.e0;
+  const T* __restrict__ pe1 = p + v.e1;
+  const T* __restrict__ pe2 = p + v.e2;
   const int j = t.j0 + t.tj, k = t.k0 + t.tk;
   const bool mine = j < v.ny && k < v.nz;
-  auto fetch_all = [&](int i, Pre& f) {
-    fetch(i, f.x, f.y, f.z);
-    f.g = (mine && i < v.nx) ? reg[v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k] : (uint8_t)0;
+  const uint8_t* __restrict__ rl = reg + v.cell_off;
+  T* __restrict__ vl = vc + 3 * v.cell_off;
+  // One step computes planes i..i+PL-1.  Its loads, issued one step ahead: the x-edges of those
+  // planes, the y/z-edges of planes i+1..i+PL and the flags of this thread's cells; planes past
+  // the block's chunk are not loaded.
+  struct Pre {
+    Prefetch<T, R> x[PL], y[PL], z[PL];
+    uint8_t g[PL];
   };
+  auto fetch = [&](int i, Pre& f) {
+#pragma unroll
+    for (int l = 0; l < PL; ++l) {
+      const int ix = i + l, iy = i + l + 1;     // block-uniform
+      const bool okx = ix < t.i1, oky = iy <= t.i1;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int q = threadIdx.x + r * 256;
+        {
+          const int jj = t.j0 + q / XK, kk = t.k0 + q % XK;
+          const bool in = okx && q < XJ * XK && jj < ny1 && kk < nz1;
+          f.x[l].v[r] = in ? pe0[(unsigned)((ix * ny1 + jj) * nz1 + kk)] : T(0);
+        }
+        {
+          const int jj = t.j0 + q / YK, kk = t.k0 + q % YK;
+          const bool in = oky && q < YJ * YK && jj < v.ny && kk < nz1;
+          f.y[l].v[r] = in ? pe1[(unsigned)((iy * v.ny + jj) * nz1 + kk)] : T(0);
+        }
+        {
+          const int jj = t.j0 + q / ZK, kk = t.k0 + q % ZK;
+          const bool in = oky && q < ZJ * ZK && jj < ny1 && kk < v.nz;
+          f.z[l].v[r] = in ? pe2[(unsigned)((iy * ny1 + jj) * v.nz + kk)] : T(0);
+        }
+      }
+      f.g[l] = (mine && okx) ? rl[(unsigned)((ix * v.ny + j) * v.nz + k)] : (uint8_t)0;
+    }
+  };
+  auto store = [&](int i, const Pre& f) {
+#pragma unroll
+    for (int l = 0; l < PL; ++l) {
+      const int b = (i + l + 1) % NS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int q = threadIdx.x + r * 256;
+        if (q < XJ * XK) sx[l][q] = f.x[l].v[r];
+        if (q < YJ * YK) sy[b][q] = f.y[l].v[r];
+        if (q < ZJ * ZK) sz[b][q] = f.z[l].v[r];
+      }
+    }
+  };
+  // y/z-edges of plane i0 (slot i0 % NS)
+  for (int q = threadIdx.x; q < YJ * YK; q += blockDim.x) {
+    const int jj = t.j0 + q / YK, kk = t.k0 + q % YK;
+    const bool in = t.i0 <= v.nx && jj < v.ny && kk < nz1;
+    sy[t.i0 % NS][q] = in ? pe1[(unsigned)((t.i0 * v.ny + jj) * nz1 + kk)] : T(0);
+  }
+  for (int q = threadIdx.x; q < ZJ * ZK; q += blockDim.x) {
+    const int jj = t.j0 + q / ZK, kk = t.k0 + q % ZK;
+    const bool in = t.i0 <= v.nx && jj < ny1 && kk < v.nz;
+    sz[t.i0 % NS][q] = in ? pe2[(unsigned)((t.i0 * ny1 + jj) * v.nz + kk)] : T(0);
+  }
   const T cf = (T)(1.0 / (2.0 * v.h * v.h));
-  auto step = [&](int i, Pre& f) {
-    const int a = i & 1, b = a ^ 1;             // slot of plane i, plane i+1
-    store(i, f.x, f.y, f.z);
-    const uint8_t g = f.g;
+  Pre f;
+  fetch(t.i0, f);
+  for (int i = t.i0; i < t.i1; i += PL) {
+    // Stores share the load counter, so the compiler waits for every load in flight before the
+    // step's first use of the buffer; one explicit wait here (vmcnt(0), the others left alone)
+    // keeps it from placing that wait after the step's own loads.
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    store(i, f);
+    uint8_t g[PL];
+#pragma unroll
+    for (int l = 0; l < PL; ++l) g[l] = f.g[l];
+    // the next step's loads reuse the registers just stored (no other wave reads them), issued
+    // before the barrier; the slots written above were last read before the previous barrier
+    if (i + PL < t.i1) fetch(i + PL, f);
     __syncthreads();
-    if (i + 2 < t.i1) fetch_all(i + 2, f);      // in flight while planes i and i+1 are computed
-    if (g) {
-      auto PX = [&](int dj, int dk) { return sx[a][(t.tj + dj) * XK + t.tk + dk]; };
-      auto PY = [&](int s, int dk) { return sy[s][t.tj * YK + t.tk + dk]; };
-      auto PZ = [&](int s, int dj) { return sz[s][(t.tj + dj) * ZK + t.tk]; };
-      const T ux0 = (PY(a, 0) - PY(a, 1)) + (PZ(a, 1) - PZ(a, 0));
-      const T ux1 = (PY(b, 0) - PY(b, 1)) + (PZ(b, 1) - PZ(b, 0));
-      const T uy0 = (PX(0, 1) - PX(0, 0)) - (PZ(b, 0) - PZ(a, 0));
-      const T uy1 = (PX(1, 1) - PX(1, 0)) - (PZ(b, 1) - PZ(a, 1));
-      const T uz0 = (PY(b, 0) - PY(a, 0)) - (PX(1, 0) - PX(0, 0));
-      const T uz1 = (PY(b, 1) - PY(a, 1)) - (PX(1, 1) - PX(0, 1));
-      const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
-      st_nt(vc + 3 * c, (ux0 + ux1) * cf);
-      st_nt(vc + 3 * c + 1, (uy0 + uy1) * cf);
-      st_nt(vc + 3 * c + 2, (uz0 + uz1) * cf);
+#pragma unroll
+    for (int l = 0; l < PL; ++l) {
+      const int ii = i + l;
+      if (ii < t.i1 && g[l]) {
+        const int a = ii % NS, b = (ii + 1) % NS;   // y/z slots of planes ii, ii+1
+        auto PX = [&](int dj, int dk) { return sx[l][(t.tj + dj) * XK + t.tk + dk]; };
+        auto PY = [&](int s, int dk) { return sy[s][t.tj * YK + t.tk + dk]; };
+        auto PZ = [&](int s, int dj) { return sz[s][(t.tj + dj) * ZK + t.tk]; };
+        const T ux0 = (PY(a, 0) - PY(a, 1)) + (PZ(a, 1) - PZ(a, 0));
+        const T ux1 = (PY(b, 0) - PY(b, 1)) + (PZ(b, 1) - PZ(b, 0));
+        const T uy0 = (PX(0, 1) - PX(0, 0)) - (PZ(b, 0) - PZ(a, 0));
+        const T uy1 = (PX(1, 1) - PX(1, 0)) - (PZ(b, 1) - PZ(a, 1));
+        const T uz0 = (PY(b, 0) - PY(a, 0)) - (PX(1, 0) - PX(0, 0));
+        const T uz1 = (PY(b, 1) - PY(a, 1)) - (PX(1, 1) - PX(0, 1));
+        const unsigned c = 3u * (unsigned)((ii * v.ny + j) * v.nz + k);
+        st_nt(vl + c, (ux0 + ux1) * cf);
+        st_nt(vl + c + 1, (uy0 + uy1) * cf);
+        st_nt(vl + c + 2, (uz0 + uz1) * cf);
+      }
     }
     __syncthreads();
-  };
-  Pre fa, fb;
-  fetch_all(t.i0, fa);
-  if (t.i0 + 1 < t.i1) fetch_all(t.i0 + 1, fb);
-  for (int i = t.i0; i < t.i1; i += 2) {
-    step(i, fa);
-    if (i + 1 < t.i1) step(i + 1, fb);
   }
 }
 
@@ -571,16 +608,21 @@ __global__ __launch_bounds__(256) void k_sg1v_t(int64_t ncell, Levels L, int ci,
       okm |= (unsigned)(q < RS && j >= 0 && j < v.ny && k >= 0 && k < v.nz) << (4 * r + 3);
       oc[r] = j * v.nz + k;
     }
-    // loads of step ip: x-edges of plane ip, y/z-edges of plane ip+1, and x of the plane-ip cells
-    // whose flag (loaded one step earlier) says it comes from HBM (bit 3: a composite G1 row)
-    auto fetch = [&](int ip, Prefetch<T, R>& fx, Prefetch<T, R>& fy, Prefetch<T, R>& fz, const uint8_t* fl,
-                     T (*xc)[3]) {
+    // loads of step ip: x-edges of plane ip, y/z-edges of plane ip+1 (fetch_edges), and x of the
+    // plane-ip cells whose flag (loaded one step earlier) says it comes from HBM (bit 3: a
+    // composite G1 row; fetch_x)
+    auto fetch_edges = [&](int ip, Prefetch<T, R>& fx, Prefetch<T, R>& fy, Prefetch<T, R>& fz) {
       const bool i_x = ip >= 0 && ip < v.nx, i_yz = ip + 1 >= 0 && ip + 1 <= v.nx;   // block-uniform
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         fx.v[r] = (i_x && (okm >> (4 * r) & 1)) ? pe0[(unsigned)(ip * PXs + ox[r])] : T(0);
         fy.v[r] = (i_yz && (okm >> (4 * r + 1) & 1)) ? pe1[(unsigned)((ip + 1) * PYs + oy[r])] : T(0);
         fz.v[r] = (i_yz && (okm >> (4 * r + 2) & 1)) ? pe2[(unsigned)((ip + 1) * PZs + oz[r])] : T(0);
+      }
+    };
+    auto fetch_x = [&](int ip, const uint8_t* fl, T (*xc)[3]) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
         if (fl[r] & 8) {                // set only inside the level box (flag loads check it)
           const unsigned c = 3u * (unsigned)(ip * PCs + oc[r]);
           xc[r][0] = xl[c];
@@ -611,29 +653,40 @@ __global__ __launch_bounds__(256) void k_sg1v_t(int64_t ncell, Levels L, int ci,
       }
     }
     Prefetch<T, R> fx, fy, fz;
-    uint8_t fl[R], fl2[R];            // flags of the plane fetched next, and of the one after
     T xc[R][3];                       // x of the fetched plane's bit-3 cells
-    fetch_flags(t.i0 - 1, fl);
-    fetch(t.i0 - 1, fx, fy, fz, fl, xc);
-    fetch_flags(t.i0, fl2);
+    uint8_t fA[R], fB[R];             // flags of two consecutive planes (the loop alternates them)
+    fetch_flags(t.i0 - 1, fA);
+    fetch_edges(t.i0 - 1, fx, fy, fz);
+    fetch_x(t.i0 - 1, fA, xc);
+    fetch_flags(t.i0, fB);
     const T cf = (T)(1.0 / (2.0 * v.h * v.h));
     const T kA = (T)v.h * visc;
     const T hv = hdt * (T)(1.0 / (v.h * v.h * v.h));
     const int j = t.j0 + t.tj, k = t.k0 + t.tk;
     const int qo = (t.tj + 1) * RK + t.tk + 1;              // this thread's cell in the region
     T xm[3] = {T(0), T(0), T(0)};                            // x of plane ip-2 at this cell
-    for (int ip = t.i0 - 1; ip <= t.i1; ++ip) {
+    // one step: fp holds the flags of plane ip and takes those of plane ip + 2; fq holds the
+    // flags of plane ip + 1.  The two arrays alternate roles from step to step, so no register
+    // that a load in flight writes is copied or overwritten.
+    auto step = [&](int ip, uint8_t* fp, const uint8_t* fq) {
       const int a = slot(ip), b = slot(ip + 1);
       uint8_t fc[R];
+      // Every load of the previous step is used now (edges and x to LDS, flags for this step's
+      // loads), so wait for all of them here (vmcnt(0); expcnt and lgkmcnt left alone).  Without
+      // it the compiler put a full wait right after this step's first edge loads, which left the
+      // prefetch one load deep.
+      __builtin_amdgcn_s_waitcnt(0x0F70);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int q = threadIdx.x + r * 256;
         if (q < XJ * XK) sx[q] = fx.v[r];
         if (q < YJ * YK) sy[b][q] = fy.v[r];
         if (q < ZJ * ZK) sz[b][q] = fz.v[r];
-        fc[r] = fl[r];
-        fl[r] = fl2[r];
+        fc[r] = fp[r];
       }
+      // the next plane's edges go into the registers just stored, before the barrier: no other
+      // wave reads them, and the slots written above were last read before the previous barrier
+      if (ip + 1 <= t.i1) fetch_edges(ip + 1, fx, fy, fz);
       __syncthreads();
       // x of the bit-3 cells goes to LDS before the registers take the next plane's loads
 #pragma unroll
@@ -645,8 +698,8 @@ __global__ __launch_bounds__(256) void k_sg1v_t(int64_t ncell, Levels L, int ci,
         }
       }
       if (ip + 1 <= t.i1) {                                  // in flight while plane ip is computed
-        fetch(ip + 1, fx, fy, fz, fl, xc);
-        if (ip + 2 <= t.i1) fetch_flags(ip + 2, fl2);
+        fetch_x(ip + 1, fq, xc);
+        if (ip + 2 <= t.i1) fetch_flags(ip + 2, fp);
       }
 #pragma unroll 1
       for (int r = 0; r < R; ++r) {                          // rolled: fewer live registers
@@ -694,15 +747,15 @@ __global__ __launch_bounds__(256) void k_sg1v_t(int64_t ncell, Levels L, int ci,
           const T nbox = (fv & 16) ? box_faces<T>(v, i, j, k) : T(0);
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
-            const T xc = sc[s0][d][qo];
-            T aa = kA * (sc[a][d][qo] - xc) + kA * (sc[s0][d][qo + RK] - xc);
-            aa = aa + kA * (sc[s0][d][qo + 1] - xc);
-            if (fv & 16) aa = aa - nbox * (kA * xc);
-            T bb = kA * (xc - xm[d]) + kA * (xc - sc[s0][d][qo - RK]);
-            bb = bb + kA * (xc - sc[s0][d][qo - 1]);
-            const T yv = rou * xc - hv * (aa - bb);
+            const T xc0 = sc[s0][d][qo];
+            T aa = kA * (sc[a][d][qo] - xc0) + kA * (sc[s0][d][qo + RK] - xc0);
+            aa = aa + kA * (sc[s0][d][qo + 1] - xc0);
+            if (fv & 16) aa = aa - nbox * (kA * xc0);
+            T bb = kA * (xc0 - xm[d]) + kA * (xc0 - sc[s0][d][qo - RK]);
+            bb = bb + kA * (xc0 - sc[s0][d][qo - 1]);
+            const T yv = rou * xc0 - hv * (aa - bb);
             y[3 * c + d] = yv;
-            dot += (double)xc * (double)yv;
+            dot += (double)xc0 * (double)yv;
           }
         }
       }
@@ -711,6 +764,10 @@ __global__ __launch_bounds__(256) void k_sg1v_t(int64_t ncell, Levels L, int ci,
 #pragma unroll
         for (int d = 0; d < 3; ++d) xm[d] = sc[s0][d][qo];
       }
+    };
+    for (int ip = t.i0 - 1; ip <= t.i1; ip += 2) {
+      step(ip, fA, fB);
+      if (ip + 1 <= t.i1) step(ip + 1, fB, fA);
     }
   }
   if (partials) {
@@ -723,7 +780,7 @@ __global__ __launch_bounds__(256) void k_sg1v_t(int64_t ncell, Levels L, int ci,
 // ---- V: y = rou x - (hdt/h^3)(a - b) on the cells of plane i (planes i-1, i, i+1 of x).
 // With partials, block partials of h^3 sum x.y over the regular cells (their share of p.A p,
 // as in k_sell_v; vol = h^3 on a regular cell)
-template <typename T>
+template <typename T, int PL>
 __device__ __forceinline__ void sv_tile(int64_t lb, const dim3& tg, int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
                                               T rou, T visc, T hdt, const T* __restrict__ x, T* __restrict__ y,
                                               double* __restrict__ partials) {
@@ -735,20 +792,27 @@ __device__ __forceinline__ void sv_tile(int64_t lb, const dim3& tg, int64_t ncel
   double dot = 0.0;
   if (t.live) {
     constexpr int PJ = TJ + 2, PK = TK + 2, PS = PJ * PK, R = 2;
-    __shared__ T sx[3][3][PS];                    // [plane slot][component][(j, k) with halo]
+    constexpr int NS = PL + 2;                    // x planes i-1..i+PL of a step
+    __shared__ T sx[NS][3][PS];                   // [plane slot][component][(j, k) with halo]
+    // 32-bit offsets inside the level's cell block from per-level bases
+    const T* __restrict__ xl = x + 3 * v.cell_off;
+    T* __restrict__ yl = y + 3 * v.cell_off;
+    const uint8_t* __restrict__ rl = reg + v.cell_off;
+    // x of plane i (0 outside the level box and past the block's last plane + 1)
     auto fetch = [&](int i, Prefetch<T, 3 * R>& f) {
+      const bool ok = i >= 0 && i < v.nx && i <= t.i1;   // block-uniform
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int q = threadIdx.x + r * 256;
         const int j = t.j0 - 1 + q / PK, k = t.k0 - 1 + q % PK;
-        const bool in = q < PS && i >= 0 && i < v.nx && j >= 0 && j < v.ny && k >= 0 && k < v.nz;
-        const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
+        const bool in = ok && q < PS && j >= 0 && j < v.ny && k >= 0 && k < v.nz;
+        const unsigned c = 3u * (unsigned)((i * v.ny + j) * v.nz + k);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) f.v[3 * r + d] = in ? x[3 * c + d] : T(0);
+        for (int d = 0; d < 3; ++d) f.v[3 * r + d] = in ? xl[c + d] : T(0);
       }
     };
     auto store = [&](int i, const Prefetch<T, 3 * R>& f) {
-      const int slot = (i + 3) % 3;
+      const int slot = (i + NS) % NS;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int q = threadIdx.x + r * 256;
@@ -763,54 +827,66 @@ __device__ __forceinline__ void sv_tile(int64_t lb, const dim3& tg, int64_t ncel
     const int j = t.j0 + t.tj, k = t.k0 + t.tk;
     const bool mine = j < v.ny && k < v.nz;
     const int q = (t.tj + 1) * PK + t.tk + 1;
-    // two planes in flight: a buffer holds plane i+1 of x and the flag of this thread's cell
-    // on plane i; buffer A serves the even steps from i0, B the odd ones
+    // One step applies V on planes i..i+PL-1.  Its loads, issued one step ahead: x of planes
+    // i+1..i+PL and the flags of this thread's cell on planes i..i+PL-1.
     struct Pre {
-      Prefetch<T, 3 * R> x;
-      uint8_t g;
+      Prefetch<T, 3 * R> x[PL];
+      uint8_t g[PL];
     };
     auto fetch_all = [&](int i, Pre& f) {
-      fetch(i, f.x);
-      f.g = (mine && i - 1 >= 0 && i - 1 < v.nx) ? reg[v.cell_off + ((int64_t)(i - 1) * v.ny + j) * v.nz + k]
-                                                  : (uint8_t)0;
-    };
-    auto step = [&](int i, Pre& f) {
-      store(i + 1, f.x);
-      const uint8_t g = f.g;
-      __syncthreads();
-      if (i + 3 <= t.i1) fetch_all(i + 3, f);   // in flight while planes i and i+1 are computed
-      if (g) {
-        const int64_t c = v.cell_off + ((int64_t)i * v.ny + j) * v.nz + k;
-        const int sm = (i + 2) % 3, s0 = i % 3, sp = (i + 1) % 3;
-        const T nbox = (g & 2) ? box_faces<T>(v, i, j, k) : T(0);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const T xc = sx[s0][d][q];
-          T a = kA * (sx[sp][d][q] - xc) + kA * (sx[s0][d][q + PK] - xc);
-          a = a + kA * (sx[s0][d][q + 1] - xc);
-          if (g & 2) a = a - nbox * (kA * xc);
-          T b = kA * (xc - sx[sm][d][q]) + kA * (xc - sx[s0][d][q - PK]);
-          b = b + kA * (xc - sx[s0][d][q - 1]);
-          const T yv = rou * xc - hv * (a - b);
-          st_nt(y + 3 * c + d, yv);
-          dot += (double)xc * (double)yv;
+      for (int l = 0; l < PL; ++l) {
+        fetch(i + l + 1, f.x[l]);
+        const int ic = i + l;
+        f.g[l] = (mine && ic < t.i1) ? rl[(unsigned)((ic * v.ny + j) * v.nz + k)] : (uint8_t)0;
+      }
+    };
+    {
+      Prefetch<T, 3 * R> f0;
+      fetch(t.i0 - 1, f0);
+      store(t.i0 - 1, f0);
+      fetch(t.i0, f0);
+      store(t.i0, f0);
+    }
+    Pre f;
+    fetch_all(t.i0, f);
+    for (int i = t.i0; i < t.i1; i += PL) {
+      // one wait for every load in flight (vmcnt(0), the others left alone): stores share the
+      // load counter, so the compiler would wait for all of them anyway, and without this it
+      // may do so right after the step's own loads
+      __builtin_amdgcn_s_waitcnt(0x0F70);
+      uint8_t g[PL];
+#pragma unroll
+      for (int l = 0; l < PL; ++l) {
+        store(i + l + 1, f.x[l]);
+        g[l] = f.g[l];
+      }
+      // the next step's loads reuse the registers just stored, before the barrier; the slots
+      // written above were last read before the previous barrier
+      if (i + PL < t.i1) fetch_all(i + PL, f);
+      __syncthreads();
+#pragma unroll
+      for (int l = 0; l < PL; ++l) {
+        const int ii = i + l;
+        if (ii < t.i1 && g[l]) {
+          const unsigned c = 3u * (unsigned)((ii * v.ny + j) * v.nz + k);
+          const int sm = (ii - 1 + NS) % NS, s0 = ii % NS, sp = (ii + 1) % NS;
+          const T nbox = (g[l] & 2) ? box_faces<T>(v, ii, j, k) : T(0);
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const T xc = sx[s0][d][q];
+            T a = kA * (sx[sp][d][q] - xc) + kA * (sx[s0][d][q + PK] - xc);
+            a = a + kA * (sx[s0][d][q + 1] - xc);
+            if (g[l] & 2) a = a - nbox * (kA * xc);
+            T b = kA * (xc - sx[sm][d][q]) + kA * (xc - sx[s0][d][q - PK]);
+            b = b + kA * (xc - sx[s0][d][q - 1]);
+            const T yv = rou * xc - hv * (a - b);
+            st_nt(yl + c + d, yv);
+            dot += (double)xc * (double)yv;
+          }
         }
       }
       __syncthreads();
-    };
-    {
-      Prefetch<T, 3 * R> f;
-      fetch(t.i0 - 1, f);
-      store(t.i0 - 1, f);
-      fetch(t.i0, f);
-      store(t.i0, f);
-    }
-    Pre fa, fb;
-    fetch_all(t.i0 + 1, fa);
-    if (t.i0 + 2 <= t.i1) fetch_all(t.i0 + 2, fb);
-    for (int i = t.i0; i < t.i1; i += 2) {
-      step(i, fa);
-      if (i + 1 < t.i1) step(i + 1, fb);
     }
   }
   if (partials) {
@@ -826,7 +902,13 @@ __device__ __forceinline__ void sv_tile(int64_t lb, const dim3& tg, int64_t ncel
 // With rr non-NULL the PCG residual update is fused in: w is not stored, r -= alpha w on the
 // same rows (alpha = scal[S_ALPHA], already final since p.A p comes from the V pass) and
 // block partials of r.prec.r go to partials[block]
+// Planes per step of the G3 tile pass (see TilePlanes).
 template <typename T>
+struct G3Planes {
+  static constexpr int value = sizeof(T) == 4 ? PYIB_G3_PL32 : PYIB_G3_PL64;
+};
+
+template <typename T, int PL>
 __device__ __forceinline__ void sg3_tile(int64_t lb, const dim3& tg, int64_t ncell, Levels L, int ci, const uint8_t* __restrict__ reg,
                                                const T* __restrict__ y, T* __restrict__ w,
                                                const double* __restrict__ scal, T* __restrict__ rr,
@@ -840,7 +922,8 @@ __device__ __forceinline__ void sg3_tile(int64_t lb, const dim3& tg, int64_t nce
   const T alf = rr ? (T)scal[S_ALPHA] : T(0);
   if (t.live) {
     constexpr int PJ = TJ + 1, PK = TK + 1, PS = PJ * PK, R = 2;   // low-side halo (j-1, k-1)
-    __shared__ T sy[2][3][PS];
+    constexpr int NS = PL + 1;                    // y planes i-1..i+PL-1 of a step
+    __shared__ T sy[NS][3][PS];
     // staged cells keep their (j, k) over the march: 32-bit in-plane offsets and in-box bits
     const int PCs = v.ny * v.nz;
     const T* __restrict__ yl = y + 3 * v.cell_off;
@@ -864,7 +947,7 @@ __device__ __forceinline__ void sg3_tile(int64_t lb, const dim3& tg, int64_t nce
       }
     };
     auto store = [&](int i, const Prefetch<T, 3 * R>& f) {
-      const int slot = (i + 2) & 1;
+      const int slot = (i + NS) % NS;             // i >= -1
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int q = threadIdx.x + r * 256;
@@ -891,14 +974,14 @@ __device__ __forceinline__ void sg3_tile(int64_t lb, const dim3& tg, int64_t nce
     const T pc = (T)v.pc;
     struct EdgeOps {
       T r[3], m[3];
-      uint8_t g[3];
+      uint32_t g[3];                     // one register each: packed bytes would make each load wait
     };
     auto efetch = [&](int i, EdgeOps& o) {
       if (!mine) return;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const int64_t e = edge(i, d);
-        o.g[d] = reg[e] & 1;
+        o.g[d] = reg[e];                 // bit 0 tested at use: testing here waits for the load
         if (rr) {
           o.r[d] = rr[e];
           if (pc == T(0)) o.m[d] = prec[e];
@@ -906,7 +989,7 @@ __device__ __forceinline__ void sg3_tile(int64_t lb, const dim3& tg, int64_t nce
       }
     };
     auto emit = [&](const EdgeOps& o, int i, int d, T val) {
-      if (!o.g[d]) return;
+      if (!(o.g[d] & 1)) return;
       const int64_t e = edge(i, d);
       if (rr) {
         const T ri = o.r[d] - alf * val;
@@ -917,43 +1000,69 @@ __device__ __forceinline__ void sg3_tile(int64_t lb, const dim3& tg, int64_t nce
         w[e] = val;
       }
     };
-    Prefetch<T, 3 * R> f;
-    EdgeOps nxt{}, cur{};
-    fetch(t.i0 - 1, f);
-    store(t.i0 - 1, f);
-    fetch(t.i0, f);
-    efetch(t.i0, nxt);
-    for (int i = t.i0; i < t.i1; ++i) {
-      store(i, f);
-      cur = nxt;
-      __syncthreads();
-      if (i + 1 < t.i1) {                          // in flight while plane i is computed
-        fetch(i + 1, f);
-        efetch(i + 1, nxt);
+    // One step computes the edges of planes i..i+PL-1.  Its loads, issued one step ahead: y of
+    // those planes and their edge operands (plane i-1 of y is in LDS from the step before).
+    struct Pre {
+      Prefetch<T, 3 * R> y[PL];
+      EdgeOps e[PL];
+    };
+    auto fetch_all = [&](int i, Pre& f) {
+#pragma unroll
+      for (int l = 0; l < PL; ++l) {
+        if (i + l < t.i1) {                        // block-uniform
+          fetch(i + l, f.y[l]);
+          efetch(i + l, f.e[l]);
+        }
       }
-      if (mine) {
-        const int s0 = i & 1, sm = s0 ^ 1;          // planes i and i-1
-        auto Y = [&](int s, int d, int dj, int dk) { return sy[s][d][q + dj * PK + dk]; };
-        {   // x-edge (i, j, k): -F_y(i,j,k) + F_y(i,j,k-1) + F_z(i,j,k) - F_z(i,j-1,k)
-          const T fy0 = hh * (Y(s0, 1, -1, 0) + Y(s0, 1, 0, 0));
-          const T fy1 = hh * (Y(s0, 1, -1, -1) + Y(s0, 1, 0, -1));
-          const T fz0 = hh * (Y(s0, 2, 0, -1) + Y(s0, 2, 0, 0));
-          const T fz1 = hh * (Y(s0, 2, -1, -1) + Y(s0, 2, -1, 0));
-          emit(cur, i, 0, (fy1 - fy0) + (fz0 - fz1));
-        }
-        {   // y-edge (i, j, k): F_x(i,j,k) - F_x(i,j,k-1) - F_z(i,j,k) + F_z(i-1,j,k)
-          const T fx0 = hh * (Y(sm, 0, 0, 0) + Y(s0, 0, 0, 0));
-          const T fx1 = hh * (Y(sm, 0, 0, -1) + Y(s0, 0, 0, -1));
-          const T fz0 = hh * (Y(s0, 2, 0, -1) + Y(s0, 2, 0, 0));
-          const T fz1 = hh * (Y(sm, 2, 0, -1) + Y(sm, 2, 0, 0));
-          emit(cur, i, 1, (fx0 - fx1) + (fz1 - fz0));
-        }
-        {   // z-edge (i, j, k): -F_x(i,j,k) + F_x(i,j-1,k) + F_y(i,j,k) - F_y(i-1,j,k)
-          const T fx0 = hh * (Y(sm, 0, 0, 0) + Y(s0, 0, 0, 0));
-          const T fx1 = hh * (Y(sm, 0, -1, 0) + Y(s0, 0, -1, 0));
-          const T fy0 = hh * (Y(s0, 1, -1, 0) + Y(s0, 1, 0, 0));
-          const T fy1 = hh * (Y(sm, 1, -1, 0) + Y(sm, 1, 0, 0));
-          emit(cur, i, 2, (fx1 - fx0) + (fy0 - fy1));
+    };
+    {
+      Prefetch<T, 3 * R> f0;
+      fetch(t.i0 - 1, f0);
+      store(t.i0 - 1, f0);
+    }
+    Pre f{};
+    fetch_all(t.i0, f);
+    for (int i = t.i0; i < t.i1; i += PL) {
+      // every load of the previous step is used now (y to LDS, edge operands): one wait for all
+      // of them (vmcnt(0); expcnt and lgkmcnt left alone), and none after this step's loads
+      __builtin_amdgcn_s_waitcnt(0x0F70);
+      EdgeOps cur[PL];
+#pragma unroll
+      for (int l = 0; l < PL; ++l) {
+        if (i + l < t.i1) store(i + l, f.y[l]);
+        cur[l] = f.e[l];
+      }
+      // the next step's loads only overwrite this thread's registers, which the lines above have
+      // read: they are issued before the barrier
+      if (i + PL < t.i1) fetch_all(i + PL, f);
+      __syncthreads();
+#pragma unroll
+      for (int l = 0; l < PL; ++l) {
+        const int ii = i + l;
+        if (ii < t.i1 && mine) {
+          const int s0 = ii % NS, sm = (ii - 1 + NS) % NS;   // planes ii and ii-1
+          auto Y = [&](int s, int d, int dj, int dk) { return sy[s][d][q + dj * PK + dk]; };
+          {   // x-edge (ii, j, k): -F_y(ii,j,k) + F_y(ii,j,k-1) + F_z(ii,j,k) - F_z(ii,j-1,k)
+            const T fy0 = hh * (Y(s0, 1, -1, 0) + Y(s0, 1, 0, 0));
+            const T fy1 = hh * (Y(s0, 1, -1, -1) + Y(s0, 1, 0, -1));
+            const T fz0 = hh * (Y(s0, 2, 0, -1) + Y(s0, 2, 0, 0));
+            const T fz1 = hh * (Y(s0, 2, -1, -1) + Y(s0, 2, -1, 0));
+            emit(cur[l], ii, 0, (fy1 - fy0) + (fz0 - fz1));
+          }
+          {   // y-edge (ii, j, k): F_x(ii,j,k) - F_x(ii,j,k-1) - F_z(ii,j,k) + F_z(ii-1,j,k)
+            const T fx0 = hh * (Y(sm, 0, 0, 0) + Y(s0, 0, 0, 0));
+            const T fx1 = hh * (Y(sm, 0, 0, -1) + Y(s0, 0, 0, -1));
+            const T fz0 = hh * (Y(s0, 2, 0, -1) + Y(s0, 2, 0, 0));
+            const T fz1 = hh * (Y(sm, 2, 0, -1) + Y(sm, 2, 0, 0));
+            emit(cur[l], ii, 1, (fx0 - fx1) + (fz1 - fz0));
+          }
+          {   // z-edge (ii, j, k): -F_x(ii,j,k) + F_x(ii,j-1,k) + F_y(ii,j,k) - F_y(ii-1,j,k)
+            const T fx0 = hh * (Y(sm, 0, 0, 0) + Y(s0, 0, 0, 0));
+            const T fx1 = hh * (Y(sm, 0, -1, 0) + Y(s0, 0, -1, 0));
+            const T fy0 = hh * (Y(s0, 1, -1, 0) + Y(s0, 1, 0, 0));
+            const T fy1 = hh * (Y(sm, 1, -1, 0) + Y(sm, 1, 0, 0));
+            emit(cur[l], ii, 2, (fx1 - fx0) + (fy0 - fy1));
+          }
         }
       }
       __syncthreads();
@@ -988,7 +1097,7 @@ __global__ __launch_bounds__(256) void k_g1_m(int64_t ntile, dim3 tg, int64_t nc
                                               const T* __restrict__ coef, int64_t ncomp, const T* __restrict__ p,
                                               T* __restrict__ vc) {
   if (blockIdx.x < ntile)
-    sg1_tile<T>(blockIdx.x, tg, ncell, L, ci, reg, p, vc);
+    sg1_tile<T, TilePlanes<T>::value>(blockIdx.x, tg, ncell, L, ci, reg, p, vc);
   else
     sell_g1_rows<T, kMergedBatch>(blockIdx.x - ntile, ncomp, nslice, row, width, sptr, col, coef, p, vc);
 }
@@ -1002,7 +1111,7 @@ __global__ __launch_bounds__(256) void k_v_m(int64_t ntile, dim3 tg, int64_t nce
                                              int64_t ncomp, const T* __restrict__ x, T* __restrict__ y,
                                              double* __restrict__ tpart, double* __restrict__ cpart) {
   if (blockIdx.x < ntile)
-    sv_tile<T>(blockIdx.x, tg, ncell, L, ci, reg, rou, visc, hdt, x, y, tpart);
+    sv_tile<T, TilePlanes<T>::value>(blockIdx.x, tg, ncell, L, ci, reg, rou, visc, hdt, x, y, tpart);
   else
     sell_v_rows<T, kMergedBatch>(blockIdx.x - ntile, ncomp, nslice, row, width, sptr, col, coef, rou, hdt * visc, x,
                                  y, rw, cpart);
@@ -1022,13 +1131,13 @@ __global__ __launch_bounds__(256) void k_g3_m(int64_t ntile, dim3 tg, int64_t nc
     // composite blocks first, padded to a multiple of 8 blocks so each tile block keeps its XCD
     const int64_t nf = (ncomp + 7) & ~int64_t(7);
     if (blockIdx.x >= nf)
-      sg3_tile<T>(blockIdx.x - nf, tg, ncell, L, ci, reg, y, w, scal, rr, prec, tpart);
+      sg3_tile<T, G3Planes<T>::value>(blockIdx.x - nf, tg, ncell, L, ci, reg, y, w, scal, rr, prec, tpart);
     else if (blockIdx.x < ncomp)
       sell_g3_rows<T, kMergedBatch>(blockIdx.x, ncomp, nslice, row, width, sptr, col, coef, mask, y, w, scal, rr,
                                     prec, rprec, cpart);
   } else {
     if (blockIdx.x < ntile)
-      sg3_tile<T>(blockIdx.x, tg, ncell, L, ci, reg, y, w, scal, rr, prec, tpart);
+      sg3_tile<T, G3Planes<T>::value>(blockIdx.x, tg, ncell, L, ci, reg, y, w, scal, rr, prec, tpart);
     else
       sell_g3_rows<T, kMergedBatch>(blockIdx.x - ntile, ncomp, nslice, row, width, sptr, col, coef, mask, y, w, scal,
                                     rr, prec, rprec, cpart);
