@@ -23,6 +23,17 @@ from .layout import (MAX_LEVELS, build_layout, composite_rows, dir_mask_internal
 G3_SIGMA = 64
 
 
+def check_level_blocks(levels):
+    """The tiled kernels address each level's cell block (3 values per cell) and edge blocks with
+    32-bit offsets from the block's base: refuse a level whose float64 block would reach 4 GiB."""
+    for i, lv in enumerate(levels):
+        nx, ny, nz = int(lv.nx), int(lv.ny), int(lv.nz)
+        largest = max(3 * nx * ny * nz, nx * (ny + 1) * (nz + 1), (nx + 1) * ny * (nz + 1), (nx + 1) * (ny + 1) * nz)
+        if 8 * largest >= 2 ** 32:
+            raise ValueError(f"level {i} ({nx}x{ny}x{nz} cells) is too large for one GPU level block: "
+                             f"{8 * largest} bytes >= 4 GiB")
+
+
 class HipGrid:
     def __init__(self, core, precisions=("float64",), margin=1, layout=None):
         self.core = core
@@ -82,6 +93,7 @@ class HipGrid:
         g.nlevels = len(lay.levels)
         if g.nlevels > MAX_LEVELS:
             raise ValueError("too many levels")
+        check_level_blocks(lay.levels)
         g.ncell, g.nedge = lay.ncell, lay.nedge
         for i, lv in enumerate(lay.levels):
             L = g.lev[i]

@@ -1,6 +1,6 @@
 """Host-side guards (no GPU): the set-up cache key and the checkpoint source identity follow every
-source that shapes the cached objects or the solution, and the HIP stencil refuses a delta
-width it cannot cover."""
+source that shapes the cached objects or the solution, the HIP stencil refuses a delta width it
+cannot cover, and the structured layout refuses a level too large for 32-bit block offsets."""
 import shutil
 
 import numpy as np
@@ -64,3 +64,13 @@ def test_setup_cache_write_failure_keeps_the_run(tmp_path, monkeypatch):
         out = SC.cached("thing", {"a": 1}, lambda: {"built": True}, dict)
     assert out == {"built": True}
     assert list(tmp_path.iterdir()) == []
+
+
+def test_level_blocks_under_4gib():
+    """The tiled kernels address a level's blocks with 32-bit offsets; the host refuses a level
+    whose float64 cell or edge block reaches 4 GiB, and accepts the largest configured one (D40)."""
+    from types import SimpleNamespace as NS
+    from pyib_hip.grid import check_level_blocks
+    check_level_blocks([NS(nx=240, ny=200, nz=200)])
+    with pytest.raises(ValueError, match="4 GiB"):
+        check_level_blocks([NS(nx=240, ny=200, nz=200), NS(nx=600, ny=600, nz=600)])
