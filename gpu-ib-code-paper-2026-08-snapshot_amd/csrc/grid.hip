@@ -401,6 +401,11 @@ struct Prefetch {
   T v[R];
 };
 
+// s_waitcnt vmcnt(0): wait until every vector memory operation this wave has issued is done,
+// leaving the export and LDS/scalar counters alone.  gfx950 (gfx9) encoding: vmcnt in bits
+// 3:0 and 15:14 (0 here), expcnt in bits 6:4 (7: no wait), lgkmcnt in bits 11:8 (15: no wait).
+__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Nontemporal store.  Measured at D32: the float32 G1 pass's x and the float32 V pass's y go out
 // nontemporally (each lowered the float32 iteration by about 6 us); in float64 the fused pass's
 // stores stay cached (nontemporal y made it 225 -> 242 us), and so do the G3 pass's r loads and
@@ -511,7 +516,7 @@ __device__ __forceinline__ void sg1_tile(int64_t lb, const dim3& tg, int64_t nce
     // Stores share the load counter, so the compiler waits for every load in flight before the
     // step's first use of the buffer; one explicit wait here (vmcnt(0), the others left alone)
     // keeps it from placing that wait after the step's own loads.
-    __builtin_amdgcn_s_waitcnt(0x0F70);
+    wait_vm_all();
     store(i, f);
     uint8_t g[PL];
 #pragma unroll
@@ -675,7 +680,7 @@ __global__ __launch_bounds__(256) void k_sg1v_t(int64_t ncell, Levels L, int ci,
       // loads), so wait for all of them here (vmcnt(0); expcnt and lgkmcnt left alone).  Without
       // it the compiler put a full wait right after this step's first edge loads, which left the
       // prefetch one load deep.
-      __builtin_amdgcn_s_waitcnt(0x0F70);
+      wait_vm_all();
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int q = threadIdx.x + r * 256;
@@ -854,7 +859,7 @@ __device__ __forceinline__ void sv_tile(int64_t lb, const dim3& tg, int64_t ncel
       // one wait for every load in flight (vmcnt(0), the others left alone): stores share the
       // load counter, so the compiler would wait for all of them anyway, and without this it
       // may do so right after the step's own loads
-      __builtin_amdgcn_s_waitcnt(0x0F70);
+      wait_vm_all();
       uint8_t g[PL];
 #pragma unroll
       for (int l = 0; l < PL; ++l) {
@@ -1026,7 +1031,7 @@ __device__ __forceinline__ void sg3_tile(int64_t lb, const dim3& tg, int64_t nce
     for (int i = t.i0; i < t.i1; i += PL) {
       // every load of the previous step is used now (y to LDS, edge operands): one wait for all
       // of them (vmcnt(0); expcnt and lgkmcnt left alone), and none after this step's loads
-      __builtin_amdgcn_s_waitcnt(0x0F70);
+      wait_vm_all();
       EdgeOps cur[PL];
 #pragma unroll
       for (int l = 0; l < PL; ++l) {
