@@ -667,6 +667,7 @@ __global__ __launch_bounds__(256) void k_sg1v_t(int64_t ncell, Levels L, int ci,
     const T cf = (T)(1.0 / (2.0 * v.h * v.h));
     const T kA = (T)v.h * visc;
     const T hv = hdt * (T)(1.0 / (v.h * v.h * v.h));
+    const T hk = hv * kA;                                    // (hdt / h^3) times the face coefficient
     const int j = t.j0 + t.tj, k = t.k0 + t.tk;
     const int qo = (t.tj + 1) * RK + t.tk + 1;              // this thread's cell in the region
     T xm[3] = {T(0), T(0), T(0)};                            // x of plane ip-2 at this cell
@@ -753,12 +754,12 @@ __global__ __launch_bounds__(256) void k_sg1v_t(int64_t ncell, Levels L, int ci,
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
             const T xc0 = sc[s0][d][qo];
-            T aa = kA * (sc[a][d][qo] - xc0) + kA * (sc[s0][d][qo + RK] - xc0);
-            aa = aa + kA * (sc[s0][d][qo + 1] - xc0);
-            if (fv & 16) aa = aa - nbox * (kA * xc0);
-            T bb = kA * (xc0 - xm[d]) + kA * (xc0 - sc[s0][d][qo - RK]);
-            bb = bb + kA * (xc0 - sc[s0][d][qo - 1]);
-            const T yv = rou * xc0 - hv * (aa - bb);
+            // a - b = kA (differences to the upper neighbours - those from the lower ones), the
+            // face coefficient kA factored out of the six faces
+            T aa = ((sc[a][d][qo] - xc0) + (sc[s0][d][qo + RK] - xc0)) + (sc[s0][d][qo + 1] - xc0);
+            if (fv & 16) aa = aa - nbox * xc0;
+            const T bb = ((xc0 - xm[d]) + (xc0 - sc[s0][d][qo - RK])) + (xc0 - sc[s0][d][qo - 1]);
+            const T yv = rou * xc0 - hk * (aa - bb);
             y[3 * c + d] = yv;
             dot += (double)xc0 * (double)yv;
           }
@@ -829,6 +830,7 @@ __device__ __forceinline__ void sv_tile(int64_t lb, const dim3& tg, int64_t ncel
     };
     const T kA = (T)v.h * visc;
     const T hv = hdt * (T)(1.0 / (v.h * v.h * v.h));
+    const T hk = hv * kA;                          // (hdt / h^3) times the face coefficient
     const int j = t.j0 + t.tj, k = t.k0 + t.tk;
     const bool mine = j < v.ny && k < v.nz;
     const int q = (t.tj + 1) * PK + t.tk + 1;
@@ -880,12 +882,11 @@ __device__ __forceinline__ void sv_tile(int64_t lb, const dim3& tg, int64_t ncel
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
             const T xc = sx[s0][d][q];
-            T a = kA * (sx[sp][d][q] - xc) + kA * (sx[s0][d][q + PK] - xc);
-            a = a + kA * (sx[s0][d][q + 1] - xc);
-            if (g[l] & 2) a = a - nbox * (kA * xc);
-            T b = kA * (xc - sx[sm][d][q]) + kA * (xc - sx[s0][d][q - PK]);
-            b = b + kA * (xc - sx[s0][d][q - 1]);
-            const T yv = rou * xc - hv * (a - b);
+            // a - b = kA (differences to the upper neighbours - those from the lower ones)
+            T a = ((sx[sp][d][q] - xc) + (sx[s0][d][q + PK] - xc)) + (sx[s0][d][q + 1] - xc);
+            if (g[l] & 2) a = a - nbox * xc;
+            const T b = ((xc - sx[sm][d][q]) + (xc - sx[s0][d][q - PK])) + (xc - sx[s0][d][q - 1]);
+            const T yv = rou * xc - hk * (a - b);
             st_nt(yl + c + d, yv);
             dot += (double)xc * (double)yv;
           }

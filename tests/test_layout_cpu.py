@@ -258,10 +258,11 @@ def _tile(lay, lv, p, flags, xmem, y, j0, k0, i0, i1, rou, visc, hdt):
             s0 = slot(i)
             for d in range(3):
                 xc = sc[s0][d][qo]
-                aa = kA * (sc[a][d][qo] - xc) + kA * (sc[s0][d][qo + RK] - xc) + kA * (sc[s0][d][qo + 1] - xc)
-                aa = aa - nbox * (kA * xc)
-                bb = kA * (xc - xm[d]) + kA * (xc - sc[s0][d][qo - RK]) + kA * (xc - sc[s0][d][qo - 1])
-                y[3 * cc[rv] + d] = (rou * xc - hv * (aa - bb))[rv]
+                # the face coefficient kA factored out of the six faces, as the kernel does
+                aa = ((sc[a][d][qo] - xc) + (sc[s0][d][qo + RK] - xc)) + (sc[s0][d][qo + 1] - xc)
+                aa = aa - nbox * xc
+                bb = ((xc - xm[d]) + (xc - sc[s0][d][qo - RK])) + (xc - sc[s0][d][qo - 1])
+                y[3 * cc[rv] + d] = (rou * xc - (hv * kA) * (aa - bb))[rv]
         if ip >= i0:
             xm = np.stack([sc[slot(ip - 1)][d][qo] for d in range(3)])
 
