@@ -537,7 +537,7 @@ def main(argv=None):
                    help="float32 residual reduction between float64 reliable updates (default: the schema's)")
     a = p.parse_args(argv)
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus})",
+        print(f"bench.py runs on one GPU: the solver has no multi-GPU path (asked for --gpus {a.gpus})",
               file=sys.stderr)
         return 2
     if a.steps < 1 or a.warmup < 1:
